@@ -1,0 +1,185 @@
+/*
+ * sep.h -- C ABI of libsep, the MI355X-native filter-diagonalization library
+ * (semi-empirical pseudopotential `filter` program: the C files of filter/ in the
+ * reference).  Plain C types only; the library owns device memory, kernels
+ * and streams, the caller owns every host array it passes.
+ *
+ * Conventions (follow the reference):
+ *   grid        x fastest: g = nx*(ny*jz + jy) + jx           (filter/init.c:138-141)
+ *   complex     interleaved {re, im} doubles                   (zomplex, filter/fd.h:17-19)
+ *   states      state-major real FP64, m x N ("psitot")        (filter/main.c:46,86)
+ *   products    every inner product is sum(...) * dv          (filter/norm.c, energy.c)
+ * Every entry point returns SEP_OK (0) or an SEP_ERR_* code and never exits
+ * the process (the reference calls nerror()/exit, filter/nerror.c:9-16);
+ * sep_last_error() describes the last failure on a context.
+ * A context is used by one host thread at a time; calls are synchronous.
+ */
+#ifndef SEP_H
+#define SEP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEP_OK 0
+#define SEP_ERR_ARG 1         /* bad argument / inconsistent sizes */
+#define SEP_ERR_UNSUPPORTED 2 /* grid length the HIP FFT has no plan for */
+#define SEP_ERR_HIP 3         /* HIP runtime error / no device */
+#define SEP_ERR_IO 4          /* missing or malformed input file */
+#define SEP_ERR_NOMEM 5       /* device or host allocation failed */
+#define SEP_ERR_NUMERIC 6     /* factorisation failed (e.g. Cholesky) */
+
+#define SEP_PREC_FP64 0 /* complex128 filter, FP64 everywhere (parity path) */
+#define SEP_PREC_FP32 1 /* complex64 filter + FP32 MFMA subspace + FP64 refinement */
+
+typedef struct sep_ctx sep_ctx;
+
+/* Grid of one nanocrystal box.  Replaces the grid part of par_st/long_st
+ * (filter/fd.h:24-37) as set up by init() (filter/init.c:95-144). */
+typedef struct {
+  long nx, ny, nz;
+  double dx, dy, dz;       /* spacing, bohr */
+  double xmin, ymin, zmin; /* origin (grid point 0), bohr */
+  double ekinmax;          /* kinetic clamp in Ha (Ekinmax = 10, filter/init.c:31) */
+  int device;              /* HIP device ordinal */
+} sep_grid_desc;
+
+/* ---------------------------------------------------------------- context */
+/* Allocates the per-grid device state (twiddles, k^2 axis tables).
+ * No counterpart function in the reference: main() allocates these in
+ * filter/main.c:27-51 and plans FFTW at filter/main.c:58-59. */
+int sep_open(const sep_grid_desc *grid, sep_ctx **ctx);
+int sep_close(sep_ctx *ctx);
+const char *sep_last_error(const sep_ctx *ctx);
+/* 1 when the library can transform a line of length n on the GPU. */
+int sep_fft_length_supported(long n);
+
+/* -------------------------------------------------------------- potential */
+/* Replaces the potl array filled by init() (filter/init.c:157-182). */
+int sep_set_potential(sep_ctx *ctx, const double *potl);
+/* Device build of V(r) = sum_a scale_a * interp_{type a}(|r - R_a|) + expot(|r|)
+ * (filter/init.c:161-182, filter/interpolate.c:11-23, expot filter/init.c:255-257;
+ * the per-atom scale is filterStrainCube/interpolate.c:20-21, 1 = plain filter).
+ * xyz: natom x 3 (bohr, already recentred as read_conf does, read.c:23-30);
+ * tables: ntype rows of `stride` doubles (vr = r grid, pot = shifted table,
+ * read.c:106), npot/dr per type (read.c:107).  expot = {e, x0, kT} or NULL.
+ * potl_out (N, optional) receives V; vmin/vmax (optional) its extrema
+ * (filter/init.c:196-199). */
+int sep_build_potential(sep_ctx *ctx, long natom, const double *xyz, const int *type,
+                        const double *scale, int ntype, const long *npot, const double *vr,
+                        const double *pot, long stride, const double *dr,
+                        const double *expot, double *potl_out, double *vmin, double *vmax);
+
+/* ---------------------------------------------------------------- H . psi */
+/* phi <- T phi + V phi on `batch` complex vectors (in != out allowed).
+ * Replaces hamiltonian(), filter/hamiltonian.c:11-23 (with kinetic(), :27-44). */
+int sep_apply_h(sep_ctx *ctx, const double *in_c128, double *out_c128, long batch);
+/* One Newton step: out = dE_1 (H in) - (2 + zm1 + vmin dE_1) in.
+ * Replaces hnorm()/hamiltonian_norm(), filter/hnorm.c:11-39. */
+int sep_hnorm(sep_ctx *ctx, const double *in_c128, double *out_c128, long batch, double zm1,
+              double vmin, double dE_1);
+/* Device-pointer variants (pointers from hipMalloc / torch): same semantics,
+ * precision SEP_PREC_FP64 = complex128, SEP_PREC_FP32 = complex64 data. */
+int sep_apply_h_dev(sep_ctx *ctx, const void *in_dev, void *out_dev, long batch, int precision);
+
+/* ---------------------------------------------------------- spectral range */
+/* Power iteration from the fixed-seed state (energy.c:57,61): returns
+ * dE = 1.1 (E - vmin) (energy.c:76), the converged E and the iteration count.
+ * Replaces get_energy_range(), filter/energy.c:53-83. */
+int sep_energy_range(sep_ctx *ctx, double vmin, double *dE, double *emax, long *iters);
+
+/* ------------------------------------------------- random states (host) */
+/* Numerical Recipes ran2 state that filter/rand.c:28-66 keeps in statics. */
+typedef struct {
+  long idum2, iy, iv[32];
+} sep_rng;
+void sep_rng_reset(sep_rng *st);
+/* ran_nrc(), filter/rand.c:28-66 (bit-exact). */
+double sep_ran_nrc(sep_rng *st, long *idum);
+/* init_psi(), filter/init.c:236-251: uniform [-1,1] real draw, normalised. */
+void sep_init_psi(sep_rng *st, long *idum, long ngrid, double dv, double *psi);
+
+/* ------------------------------------------- Newton coefficients (host) */
+/* samp_points_ashkenazy(), filter/coeff.c:82-134: nc Leja-type points on
+ * [-2,2]; returns the capacity estimate fk (coeff.c:295-297). */
+double sep_leja_points(long nc, double *zn);
+/* coefficient(), filter/coeff.c:16-63: an (ms x nc, real parts; the
+ * reference's imaginary parts are identically 0) of the Gaussian
+ * sqrt(dt/pi) exp(-dt (E - El)^2) at the points mapped to [vmin, vmin + dE]. */
+int sep_newton_coeffs(long nc, long ms, const double *zn, const double *el, double dE,
+                      double vmin, double dt, double *an);
+
+/* ------------------------------------------------------------ the filter */
+/* Filters ns real random states (ns x N, each normalised) for ms targets:
+ * psitot[jns*ms + ie] = sum_j an[ie][j] Re psi_j(jns), psi_j = prod (X - z_k) psi_0,
+ * X = (H - vmin) 4/dE - 2, then normalize_all (filter.c:33).
+ * Replaces filtering()/filter(), filter/filter.c:11-83 (the OpenMP loop of
+ * filter/main.c:92-98 becomes one batched launch sequence).  The result
+ * stays resident in the context (see sep_states_*); psitot (optional,
+ * (ns*ms) x N host) receives a copy and ene (optional, ns*ms) the Rayleigh
+ * quotients of energy_all (filter.c:36, energy.c:28-49).
+ * precision: SEP_PREC_FP64 or SEP_PREC_FP32 (complex64 recurrence/FFTs,
+ * FP64 accumulators). */
+int sep_filter(sep_ctx *ctx, const double *psi0, long ns, const double *an, const double *zn,
+               long nc, long ms, double vmin, double dE, int precision, double *psitot,
+               double *ene);
+
+/* Same as sep_filter with the random states already on the device
+ * (psi0_dev: ns x N FP64, hipMalloc'd); nothing is copied back.  This is the
+ * entry a caller uses when the inputs are resident in HBM (no counterpart
+ * in the reference, whose states always live in host memory). */
+int sep_filter_dev(sep_ctx *ctx, const void *psi0_dev, long ns, const double *an,
+                   const double *zn, long nc, long ms, double vmin, double dE, int precision);
+
+/* ------------------------------------------------- dense device contractions */
+/* On device buffers of the context's grid (N = nx ny nz points per state,
+ * state-major; FP64 when precision = SEP_PREC_FP64, FP32 otherwise), on the
+ * matrix cores:
+ *   sep_gram_dev    C[i][j] = sum_g A[i][g] B[j][g]   (C: host, m x mb, FP64)
+ *   sep_rotate_dev  Out[j][g] = sum_i M[i][j] A[i][g] (M: host, m x mo row-major)
+ * They are the projected-H / overlap contractions and the Ritz rotation of
+ * filter/Hmat.c:30-46 and :61-78 (there scalar loops). */
+int sep_gram_dev(sep_ctx *ctx, int precision, const void *A_dev, long m, const void *B_dev,
+                 long mb, double *C);
+int sep_rotate_dev(sep_ctx *ctx, int precision, const void *A_dev, long m, const double *M,
+                   long mo, void *Out_dev);
+
+/* ------------------------------------------------------------ measurement */
+/* Times the H.psi passes on nvec resident complex vectors (random data) with
+ * HIP events on the context's stream: kernel_ms[0..3] = average duration of
+ * pass A (xy forward), pass B (z pencils with k^2), pass C (xy backward +
+ * Newton epilogue) and one accumulation launch (K = 64 Newton vectors,
+ * ms targets, 2 nvec real states); kernel_ms[4] = passes A+B+C timed as one
+ * span.  No counterpart in the reference (it only prints clock() totals,
+ * filter/main.c:63-66,91-100). */
+int sep_bench_hpsi(sep_ctx *ctx, int precision, long nvec, long ms, long iters,
+                   double *kernel_ms);
+
+/* -------------------------------------------------- resident state block */
+/* Upload/download m states (m x N FP64) to/from the context. */
+int sep_states_upload(sep_ctx *ctx, const double *psi, long m);
+int sep_states_download(sep_ctx *ctx, double *psi, long m);
+/* normalize_all(), filter/norm.c:40-54, on the first m resident states. */
+int sep_normalize_all(sep_ctx *ctx, long m);
+/* energy_all(), filter/energy.c:28-49: ene[i] = <psi_i|H|psi_i> dv. */
+int sep_energy_all(sep_ctx *ctx, long m, double *ene);
+
+/* ---------------------------------------------------------------- subspace */
+/* Orthonormalises the m resident states (thin SVD, shifted Cholesky-QR +
+ * one-sided Jacobi on R), keeps the columns with S_i/S_0 >= cutoff (1e-10 in
+ * the reference), then normalize_all.  Replaces portho(), filter/ortho.c:11-52
+ * (+ normalize_all at filter/main.c:112).  S (optional, m) gets the singular
+ * values in descending order, *kept the new m. */
+int sep_ortho(sep_ctx *ctx, long m, double cutoff, double *S, long *kept);
+/* Rayleigh-Ritz on the m resident (orthonormal) states: H_ij = dv <psi_i|H psi_j>,
+ * eigen-decomposition, psi <- psi Q, normalize_all; eval ascending.
+ * Replaces Hmatreal(), filter/Hmat.c:11-86 (+ normalize_all, main.c:122). */
+int sep_rayleigh_ritz(sep_ctx *ctx, long m, double *eval);
+/* sigma_n = sqrt|<psi|H^2 psi> dv - (<psi|H psi> dv)^2| of the m resident states.
+ * Replaces calc_sigma_E(), filter/energy.c:88-119. */
+int sep_sigma(sep_ctx *ctx, long m, double *sige);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,109 @@
+/*
+ * sep_oracle.h -- CPU restatement (FP64, OpenMP) of the reference `filter`
+ * program's filter-diagonalization path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the libsep library, the
+ * filter driver, the Python wrapper) links, loads or calls this code.  It is
+ * used by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+ * leg, always as the checker (or the timed CPU baseline), never as the thing
+ * shipped.
+ *
+ * Every function cites the reference file:line it follows (paths relative to
+ * the reference repository root).  Arithmetic order follows the reference
+ * source as written (compiled with -ffp-contract=off), so that, for the same
+ * libm, results are the ones the reference source defines.
+ */
+#ifndef SEP_ORACLE_H
+#define SEP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- FFT (replaces FFTW3 at filter/hamiltonian.c:32,39) ---------------- */
+/* In-place unnormalised 3D complex DFT of an x-fastest (nz,ny,nx) volume of
+ * interleaved complex doubles; sign -1 = FFTW_FORWARD, +1 = FFTW_BACKWARD. */
+int or_fft3d(long nx, long ny, long nz, double *data, int sign);
+/* 1D transform of one contiguous line (for tests). */
+int or_fft1d(long n, double *data, int sign);
+
+/* ---- grid / operator setup (filter/init.c) ------------------------------ */
+/* ksqr exactly as filter/init.c:125-144 (1/N and the Ekinmax clamp folded). */
+int or_ksqr(long nx, long ny, long nz, double dx, double dy, double dz,
+            double ekinmax, double *ksqr);
+/* V(r) = sum_atoms interp + expot, filter/init.c:157-182,255-257 with the
+ * per-atom scale of filterStrainCube/interpolate.c:11-23 (scale 1 = plain). */
+int or_build_potential(long nx, long ny, long nz, double xmin, double ymin,
+                       double zmin, double dx, double dy, double dz,
+                       long natom, const double *rx, const double *ry,
+                       const double *rz, const long *type, const double *scale,
+                       const long *npot, const double *vr, const double *pot,
+                       long stride, const double *dr, double ex_e,
+                       double ex_x0, double ex_t, double *potl);
+/* interpolate(), filter/interpolate.c:11-23 (+ strain scale). */
+double or_interpolate(double r, double dr, const double *vr, const double *pot,
+                      long stride, long n, long j, double scale);
+
+/* ---- H application (filter/hamiltonian.c:11-44) ------------------------- */
+/* phi <- T phi + V phi for `batch` complex vectors of length N. */
+int or_hamiltonian(long nx, long ny, long nz, const double *ksqr,
+                   const double *potl, double *phi, long batch);
+
+/* ---- random states (filter/rand.c:28-66, filter/init.c:236-251) --------- */
+typedef struct {
+  long idum2, iy, iv[32];
+} or_rng;
+void or_rng_reset(or_rng *st);
+double or_ran_nrc(or_rng *st, long *idum);
+/* init_psi: real uniform [-1,1] draw then normalise (sum psi^2 dv = 1). */
+void or_init_psi(or_rng *st, long *idum, long ngrid, double dv, double *psi_re);
+
+/* ---- spectral range (filter/energy.c:53-83) ----------------------------- */
+int or_energy_range(long nx, long ny, long nz, const double *ksqr,
+                    const double *potl, double dv, double vmin, double *dE,
+                    double *emax, long *iters);
+
+/* ---- Newton coefficients (filter/coeff.c:16-134) ------------------------ */
+double or_leja_points(long nc, double *zn);
+int or_newton_coeffs(long nc, long ms, const double *zn, const double *el,
+                     double dE, double vmin, double dt, double *an);
+
+/* ---- filter (filter/filter.c:59-83, hnorm.c:11-39) ---------------------- */
+/* psi0: the real random state (length N); out: ms x N real filtered states. */
+int or_filter_state(long nx, long ny, long nz, const double *ksqr,
+                    const double *potl, const double *an, const double *zn,
+                    long nc, long ms, double dE_1, double vmin,
+                    const double *psi0, double *out);
+
+/* ---- norms / energies (filter/norm.c:40-54, filter/energy.c:28-49) ------ */
+void or_normalize_all(double *psi, double dv, long m, long ngrid);
+int or_energy_all(long nx, long ny, long nz, const double *ksqr,
+                  const double *potl, double dv, const double *psi, long m,
+                  double *ene);
+
+/* ---- subspace (filter/ortho.c:11-52, Hmat.c:11-99, energy.c:88-119) ----- */
+/* Thin SVD by Householder QR + one-sided Jacobi on R.  Overwrites psi (N x m
+ * column-major = state-major) with the left singular vectors, S descending.
+ * Returns the reference cutoff (first i with S_i/S_0 < 1e-10). */
+long or_ortho(double *psi, long ngrid, long m, double *S);
+int or_hmat(long nx, long ny, long nz, const double *ksqr, const double *potl,
+            double dv, double *psi, long m, double *eval);
+int or_sigma(long nx, long ny, long nz, const double *ksqr, const double *potl,
+             double dv, const double *psi, long m, double *sige);
+
+/* ---- small dense FP64 linear algebra (replaces MKL dsyev_) -------------- */
+/* Cyclic Jacobi: A (n x n row-major symmetric) -> w ascending, V columns. */
+int or_jacobi_eig(long n, const double *A, double *w, double *V);
+
+/* ---- whole program (filter/main.c:11-162) ------------------------------- */
+/* Runs the reference pipeline on the deck in `indir` (input.par, conf.par,
+ * pot*.par, optional expot.par / strain.par), writes eval.dat, psi.dat,
+ * eval-filt-0-<jns>.dat and oracle_summary.dat into `outdir`.
+ * idum: the (negative) seed that filter/main.c:79 draws from the clock. */
+int or_run(const char *indir, const char *outdir, long idum, long nthreads);
+const char *or_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,380 @@
+"""ctypes binding of libsep (include/sep.h).
+
+The Python side is a thin host mirror of the C ABI; every numerical call runs
+in libsep's HIP kernels.  There is no CPU fallback: importing works anywhere,
+but constructing a Context without the built library or without a GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsep.so")
+FILTER_BIN = os.path.join(_HERE, "bin", "filter")
+
+SEP_OK = 0
+SEP_PREC_FP64 = 0
+SEP_PREC_FP32 = 1
+_ERRNAMES = {1: "SEP_ERR_ARG", 2: "SEP_ERR_UNSUPPORTED", 3: "SEP_ERR_HIP", 4: "SEP_ERR_IO",
+             5: "SEP_ERR_NOMEM", 6: "SEP_ERR_NUMERIC"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+_ip = ctypes.POINTER(ctypes.c_int)
+_L = ctypes.c_long
+_D = ctypes.c_double
+_I = ctypes.c_int
+_V = ctypes.c_void_p
+
+
+class SepError(RuntimeError):
+    pass
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("nx", _L), ("ny", _L), ("nz", _L), ("dx", _D), ("dy", _D), ("dz", _D),
+                ("xmin", _D), ("ymin", _D), ("zmin", _D), ("ekinmax", _D), ("device", _I)]
+
+
+NTYPE = 17
+NPOT = 8192
+
+
+class Deck(ctypes.Structure):
+    """sep_deck (include/sep_deck.h)."""
+    _fields_ = [("nx", _L), ("ny", _L), ("nz", _L), ("ms", _L), ("ns", _L), ("nc", _L),
+                ("nthreads", _L), ("Elmin", _D), ("Elmax", _D), ("natom", _L),
+                ("xyz", _dp), ("type", _ip), ("sym", ctypes.c_void_p), ("scale", _dp),
+                ("has_strain", _I), ("expot", _D * 3), ("has_expot", _I),
+                ("npot", _L * NTYPE), ("dr", _D * NTYPE), ("vr", _dp), ("pot", _dp),
+                ("xd", _D), ("yd", _D), ("zd", _D), ("xmin", _D), ("ymin", _D), ("zmin", _D),
+                ("dx", _D), ("dy", _D), ("dz", _D), ("dv", _D), ("seed", _L), ("has_seed", _I)]
+
+
+class Rng(ctypes.Structure):
+    _fields_ = [("idum2", _L), ("iy", _L), ("iv", _L * 32)]
+
+
+# Exported symbols and their signatures (restype, argtypes); the loader
+# checks every one of them, the tests check the list against include/*.h.
+SIGNATURES = {
+    "sep_open": (_I, [ctypes.POINTER(GridDesc), ctypes.POINTER(_V)]),
+    "sep_close": (_I, [_V]),
+    "sep_last_error": (ctypes.c_char_p, [_V]),
+    "sep_fft_length_supported": (_I, [_L]),
+    "sep_set_potential": (_I, [_V, _dp]),
+    "sep_build_potential": (_I, [_V, _L, _dp, _ip, _dp, _I, _lp, _dp, _dp, _L, _dp, _dp, _dp,
+                                 _dp, _dp]),
+    "sep_apply_h": (_I, [_V, _dp, _dp, _L]),
+    "sep_hnorm": (_I, [_V, _dp, _dp, _L, _D, _D, _D]),
+    "sep_apply_h_dev": (_I, [_V, _V, _V, _L, _I]),
+    "sep_energy_range": (_I, [_V, _D, _dp, _dp, _lp]),
+    "sep_rng_reset": (None, [ctypes.POINTER(Rng)]),
+    "sep_ran_nrc": (_D, [ctypes.POINTER(Rng), _lp]),
+    "sep_init_psi": (None, [ctypes.POINTER(Rng), _lp, _L, _D, _dp]),
+    "sep_leja_points": (_D, [_L, _dp]),
+    "sep_newton_coeffs": (_I, [_L, _L, _dp, _dp, _D, _D, _D, _dp]),
+    "sep_filter": (_I, [_V, _dp, _L, _dp, _dp, _L, _L, _D, _D, _I, _dp, _dp]),
+    "sep_states_upload": (_I, [_V, _dp, _L]),
+    "sep_states_download": (_I, [_V, _dp, _L]),
+    "sep_normalize_all": (_I, [_V, _L]),
+    "sep_energy_all": (_I, [_V, _L, _dp]),
+    "sep_ortho": (_I, [_V, _L, _D, _dp, _lp]),
+    "sep_rayleigh_ritz": (_I, [_V, _L, _dp]),
+    "sep_sigma": (_I, [_V, _L, _dp]),
+    "sep_filter_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D, _I]),
+    "sep_bench_hpsi": (_I, [_V, _I, _L, _L, _L, _dp]),
+    "sep_gram_dev": (_I, [_V, _I, _V, _L, _V, _L, _dp]),
+    "sep_rotate_dev": (_I, [_V, _I, _V, _L, _dp, _L, _V]),
+    "sep_deck_read": (_I, [ctypes.c_char_p, _V, ctypes.c_char_p, _I]),
+    "sep_deck_free": (None, [_V]),
+    "sep_type_symbol": (ctypes.c_char_p, [_I]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libsep.so (raises if it has not been built).
+
+    torch, when importable, is imported first: its wheel bundles a HIP runtime
+    with the same soname as ROCm's, and loading it first makes libsep and
+    torch share one runtime in the process (torch is only used for device
+    buffers by callers such as the autograd wrapper)."""
+    global _lib
+    if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise SepError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                           "(make -C semi_empirical_pseudopotentials_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------ host helpers
+def ran_nrc_stream(idum, count, rng=None):
+    """`count` draws of ran_nrc (filter/rand.c:28-66); returns (values, idum, rng)."""
+    L = lib()
+    if rng is None:
+        rng = Rng()
+        L.sep_rng_reset(ctypes.byref(rng))
+    idp = np.array([idum], dtype=np.int64)
+    out = np.empty(count)
+    for i in range(count):
+        out[i] = L.sep_ran_nrc(ctypes.byref(rng), idp.ctypes.data_as(_lp))
+    return out, int(idp[0]), rng
+
+
+def init_psi(idum, ngrid, dv, rng=None):
+    """init_psi (filter/init.c:236-251); returns (psi, idum, rng)."""
+    L = lib()
+    if rng is None:
+        rng = Rng()
+        L.sep_rng_reset(ctypes.byref(rng))
+    idp = np.array([idum], dtype=np.int64)
+    out = np.empty(ngrid)
+    L.sep_init_psi(ctypes.byref(rng), idp.ctypes.data_as(_lp), ngrid, dv, _p(out))
+    return out, int(idp[0]), rng
+
+
+def leja_points(nc):
+    zn = np.empty(nc)
+    fk = lib().sep_leja_points(nc, _p(zn))
+    return zn, fk
+
+
+def newton_coeffs(zn, el, dE, vmin, dt):
+    zn = _f64(zn)
+    el = _f64(el)
+    an = np.empty((el.size, zn.size))
+    rc = lib().sep_newton_coeffs(zn.size, el.size, _p(zn), _p(el), dE, vmin, dt, _p(an))
+    if rc != SEP_OK:
+        raise SepError(f"sep_newton_coeffs: {_ERRNAMES.get(rc, rc)}")
+    return an
+
+
+def read_deck(path):
+    """Parse a deck directory with libsep's reader; returns a dict of numpy arrays."""
+    L = lib()
+    d = Deck()
+    err = ctypes.create_string_buffer(512)
+    rc = L.sep_deck_read(path.encode(), ctypes.byref(d), err, 512)
+    if rc != SEP_OK:
+        raise SepError(f"sep_deck_read: {err.value.decode()}")
+    na = d.natom
+    out = {k: getattr(d, k) for k in ("nx", "ny", "nz", "ms", "ns", "nc", "nthreads", "Elmin",
+                                        "Elmax", "natom", "has_strain", "has_expot", "xd", "yd",
+                                        "zd", "xmin", "ymin", "zmin", "dx", "dy", "dz", "dv",
+                                        "seed", "has_seed")}
+    out["xyz"] = np.ctypeslib.as_array(d.xyz, shape=(na, 3)).copy()
+    out["type"] = np.ctypeslib.as_array(d.type, shape=(na,)).copy()
+    out["scale"] = np.ctypeslib.as_array(d.scale, shape=(na,)).copy()
+    out["expot"] = np.array(d.expot[:])
+    out["npot"] = np.array(d.npot[:], dtype=np.int64)
+    out["dr"] = np.array(d.dr[:])
+    out["vr"] = np.ctypeslib.as_array(d.vr, shape=(NTYPE, NPOT)).copy()
+    out["pot"] = np.ctypeslib.as_array(d.pot, shape=(NTYPE, NPOT)).copy()
+    L.sep_deck_free(ctypes.byref(d))
+    return out
+
+
+def fft_length_supported(n):
+    return bool(lib().sep_fft_length_supported(n))
+
+
+class Context:
+    """One grid on one GPU (sep_open / sep_close)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, xmin=0.0, ymin=0.0, zmin=0.0, ekinmax=10.0,
+                 device=0):
+        L = lib()
+        self.grid = (nx, ny, nz)
+        self.N = nx * ny * nz
+        self.dv = dx * dy * dz
+        self._desc = GridDesc(nx, ny, nz, dx, dy, dz, xmin, ymin, zmin, ekinmax, device)
+        h = _V()
+        rc = L.sep_open(ctypes.byref(self._desc), ctypes.byref(h))
+        if rc != SEP_OK:
+            raise SepError(f"sep_open failed: {_ERRNAMES.get(rc, rc)}")
+        self._h = h
+        self.vmin = None
+        self.vmax = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sep_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != SEP_OK:
+            msg = lib().sep_last_error(self._h)
+            raise SepError(f"{what}: {_ERRNAMES.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    # -------------------------------------------------------- potential
+    def set_potential(self, potl):
+        potl = _f64(potl).reshape(-1)
+        assert potl.size == self.N
+        self._chk(lib().sep_set_potential(self._h, _p(potl)), "sep_set_potential")
+        self.vmin, self.vmax = float(potl.min()), float(potl.max())
+
+    def build_potential(self, xyz, types, npot, vr, pot, dr, scale=None, expot=None):
+        xyz = _f64(xyz).reshape(-1, 3)
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        npot = np.ascontiguousarray(npot, dtype=np.int64)
+        vr = _f64(vr)
+        pot = _f64(pot)
+        dr = _f64(dr)
+        out = np.empty(self.N)
+        vmin = np.zeros(1)
+        vmax = np.zeros(1)
+        sc = _p(_f64(scale)) if scale is not None else None
+        ex = _p(_f64(expot)) if expot is not None else None
+        self._keep = (scale if scale is None else _f64(scale))
+        rc = lib().sep_build_potential(self._h, xyz.shape[0], _p(xyz), t.ctypes.data_as(_ip), sc,
+                                       vr.shape[0], npot.ctypes.data_as(_lp), _p(vr), _p(pot),
+                                       vr.shape[1], _p(dr), ex, _p(out), _p(vmin), _p(vmax))
+        self._chk(rc, "sep_build_potential")
+        self.vmin, self.vmax = float(vmin[0]), float(vmax[0])
+        return out
+
+    # ------------------------------------------------------------ H psi
+    def apply_h(self, psi):
+        """H on complex128 vectors shaped (batch, N) or (N,)."""
+        v = np.ascontiguousarray(psi, dtype=np.complex128)
+        shape = v.shape
+        v = v.reshape(-1, self.N)
+        out = np.empty_like(v)
+        rc = lib().sep_apply_h(self._h, v.ctypes.data_as(_dp), out.ctypes.data_as(_dp), v.shape[0])
+        self._chk(rc, "sep_apply_h")
+        return out.reshape(shape)
+
+    def hnorm(self, psi, zm1, vmin, dE_1):
+        v = np.ascontiguousarray(psi, dtype=np.complex128)
+        shape = v.shape
+        v = v.reshape(-1, self.N)
+        out = np.empty_like(v)
+        rc = lib().sep_hnorm(self._h, v.ctypes.data_as(_dp), out.ctypes.data_as(_dp), v.shape[0],
+                             zm1, vmin, dE_1)
+        self._chk(rc, "sep_hnorm")
+        return out.reshape(shape)
+
+    def apply_h_dev(self, in_ptr, out_ptr, batch, precision=SEP_PREC_FP64):
+        rc = lib().sep_apply_h_dev(self._h, _V(in_ptr), _V(out_ptr), batch, precision)
+        self._chk(rc, "sep_apply_h_dev")
+
+    def energy_range(self, vmin=None):
+        vmin = self.vmin if vmin is None else vmin
+        dE = np.zeros(1)
+        emax = np.zeros(1)
+        it = np.zeros(1, dtype=np.int64)
+        rc = lib().sep_energy_range(self._h, vmin, _p(dE), _p(emax), it.ctypes.data_as(_lp))
+        self._chk(rc, "sep_energy_range")
+        return float(dE[0]), float(emax[0]), int(it[0])
+
+    # ------------------------------------------------------------ filter
+    def filter(self, psi0, an, zn, vmin, dE, precision=SEP_PREC_FP64, want_states=True,
+               want_energies=True):
+        psi0 = _f64(psi0).reshape(-1, self.N)
+        an = _f64(an)
+        zn = _f64(zn)
+        ns = psi0.shape[0]
+        ms, nc = an.shape
+        out = np.empty((ns * ms, self.N)) if want_states else None
+        ene = np.empty(ns * ms) if want_energies else None
+        rc = lib().sep_filter(self._h, _p(psi0), ns, _p(an), _p(zn), nc, ms, vmin, dE, precision,
+                              _p(out) if out is not None else None,
+                              _p(ene) if ene is not None else None)
+        self._chk(rc, "sep_filter")
+        return out, ene
+
+    def filter_dev(self, psi0_ptr, ns, an, zn, vmin, dE, precision=SEP_PREC_FP64):
+        """Filter with the random states already on the device (ns x N FP64)."""
+        an = _f64(an)
+        zn = _f64(zn)
+        ms, nc = an.shape
+        rc = lib().sep_filter_dev(self._h, _V(psi0_ptr), ns, _p(an), _p(zn), nc, ms, vmin, dE,
+                                  precision)
+        self._chk(rc, "sep_filter_dev")
+
+    def bench_hpsi(self, precision, nvec, ms, iters):
+        out = np.zeros(5)
+        rc = lib().sep_bench_hpsi(self._h, precision, nvec, ms, iters, _p(out))
+        self._chk(rc, "sep_bench_hpsi")
+        return out
+
+    def gram_dev(self, A_ptr, m, B_ptr, mb, precision=SEP_PREC_FP64):
+        C = np.empty((m, mb))
+        rc = lib().sep_gram_dev(self._h, precision, _V(A_ptr), m, _V(B_ptr), mb, _p(C))
+        self._chk(rc, "sep_gram_dev")
+        return C
+
+    def rotate_dev(self, A_ptr, m, M, out_ptr, precision=SEP_PREC_FP64):
+        M = _f64(M)
+        rc = lib().sep_rotate_dev(self._h, precision, _V(A_ptr), m, _p(M), M.shape[1],
+                                  _V(out_ptr))
+        self._chk(rc, "sep_rotate_dev")
+
+    # ------------------------------------------------------------ states
+    def states_upload(self, psi):
+        psi = _f64(psi).reshape(-1, self.N)
+        self._chk(lib().sep_states_upload(self._h, _p(psi), psi.shape[0]), "sep_states_upload")
+        return psi.shape[0]
+
+    def states_download(self, m):
+        out = np.empty((m, self.N))
+        self._chk(lib().sep_states_download(self._h, _p(out), m), "sep_states_download")
+        return out
+
+    def normalize_all(self, m):
+        self._chk(lib().sep_normalize_all(self._h, m), "sep_normalize_all")
+
+    def energy_all(self, m):
+        e = np.empty(m)
+        self._chk(lib().sep_energy_all(self._h, m, _p(e)), "sep_energy_all")
+        return e
+
+    def ortho(self, m, cutoff=1.0e-10):
+        S = np.empty(m)
+        kept = np.zeros(1, dtype=np.int64)
+        rc = lib().sep_ortho(self._h, m, cutoff, _p(S), kept.ctypes.data_as(_lp))
+        self._chk(rc, "sep_ortho")
+        return int(kept[0]), S
+
+    def rayleigh_ritz(self, m):
+        ev = np.empty(m)
+        self._chk(lib().sep_rayleigh_ritz(self._h, m, _p(ev)), "sep_rayleigh_ritz")
+        return ev
+
+    def sigma(self, m):
+        s = np.empty(m)
+        self._chk(lib().sep_sigma(self._h, m, _p(s)), "sep_sigma")
+        return s

@@ -1,0 +1,818 @@
+// sep_api.cpp -- the C ABI of libsep (include/sep.h): context, device memory
+// and the host-side orchestration of the filter-diagonalization pipeline.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fft_core.h"
+#include "sep_internal.h"
+
+using namespace sep;
+
+struct sep_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  GridDev gd{};
+  double dx = 0, dy = 0, dz = 0, dv = 0, xmin = 0, ymin = 0, zmin = 0, ekinmax = 10.0;
+  double *pot64 = nullptr;
+  float *pot32 = nullptr;
+  bool has_pot = false;
+  double vmin = 0, vmax = 0;
+  // resident state block (FP64, m x N) and a second block of the same shape
+  double *states = nullptr, *states2 = nullptr;
+  long cap1 = 0, cap2 = 0;
+  // FFT intermediate and general scratch
+  void *tb = nullptr;
+  size_t tb_bytes = 0;
+  double *scr = nullptr;
+  size_t scr_len = 0;
+  void *ring = nullptr;
+  size_t ring_bytes = 0;
+  std::string err;
+};
+
+static int fail(sep_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, x)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail((c), SEP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+#define RCHK(x)               \
+  do {                        \
+    int r_ = (x);             \
+    if (r_ != SEP_OK) return r_; \
+  } while (0)
+
+static int ensure(sep_ctx *c, void **p, size_t *cap, size_t bytes) {
+  if (*cap >= bytes && *p) return SEP_OK;
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (bytes == 0) return SEP_OK;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(c, SEP_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  *cap = bytes;
+  return SEP_OK;
+}
+
+static int ensure_states(sep_ctx *c, long m, int which) {
+  size_t bytes = sizeof(double) * (size_t)m * c->gd.N;
+  size_t cap = sizeof(double) * (size_t)(which == 1 ? c->cap1 : c->cap2) * c->gd.N;
+  void **p = (void **)(which == 1 ? &c->states : &c->states2);
+  if (cap >= bytes && *p) return SEP_OK;
+  RCHK(ensure(c, p, &cap, bytes));
+  (which == 1 ? c->cap1 : c->cap2) = m;
+  return SEP_OK;
+}
+static int ensure_scr(sep_ctx *c, size_t len) {
+  size_t cap = c->scr_len * sizeof(double);
+  RCHK(ensure(c, (void **)&c->scr, &cap, len * sizeof(double)));
+  if (len > c->scr_len) c->scr_len = len;
+  return SEP_OK;
+}
+
+extern "C" {
+
+const char *sep_last_error(const sep_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int sep_fft_length_supported(long n) { return hop_supported((int)n) ? 1 : 0; }
+
+int sep_open(const sep_grid_desc *g, sep_ctx **out) {
+  if (!g || !out) return SEP_ERR_ARG;
+  *out = nullptr;
+  for (long n : {g->nx, g->ny, g->nz})
+    if (!hop_supported((int)n)) return SEP_ERR_UNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= g->device) return SEP_ERR_HIP;
+  sep_ctx *c = new sep_ctx();
+  c->device = g->device;
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+    delete c;
+    return SEP_ERR_HIP;
+  }
+  c->gd.nx = (int)g->nx;
+  c->gd.ny = (int)g->ny;
+  c->gd.nz = (int)g->nz;
+  c->gd.N = g->nx * g->ny * g->nz;
+  c->dx = g->dx;
+  c->dy = g->dy;
+  c->dz = g->dz;
+  c->dv = g->dx * g->dy * g->dz;
+  c->xmin = g->xmin;
+  c->ymin = g->ymin;
+  c->zmin = g->zmin;
+  c->ekinmax = g->ekinmax > 0 ? g->ekinmax : 10.0;
+  // k^2 axis tables as filter/init.c:124-137 (1/N folded in, reference order)
+  const double TWOPI = 6.28318530717958647692;
+  const long n3[3] = {g->nx, g->ny, g->nz};
+  const double d3[3] = {g->dx, g->dy, g->dz};
+  const double nx_1 = 1.0 / (double)g->nx, ny_1 = 1.0 / (double)g->ny, nz_1 = 1.0 / (double)g->nz;
+  double *tabs_d[3] = {nullptr, nullptr, nullptr};
+  for (int a = 0; a < 3; a++) {
+    const long n = n3[a];
+    std::vector<double> k(n, 0.0);
+    const double dk = TWOPI / ((double)n * d3[a]);
+    for (long j = 1; j <= n / 2; j++) {
+      double q = (double)j * dk;
+      double v;
+      if (a == 0)
+        v = 0.5 * (q * q) * nx_1 * ny_1 * nz_1 / 1.0;
+      else if (a == 1)
+        v = 0.5 * (q * q) * ny_1 * nx_1 * nz_1 / 1.0;
+      else
+        v = 0.5 * (q * q) * nz_1 * nx_1 * ny_1 / 1.0;
+      k[j] = k[n - j] = v;
+    }
+    if (hipMalloc(&tabs_d[a], sizeof(double) * n) != hipSuccess ||
+        hipMemcpy(tabs_d[a], k.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) {
+      sep_close(c);
+      return SEP_ERR_HIP;
+    }
+    // twiddles exp(-2 pi i m / n)
+    std::vector<double> t64(2 * n);
+    std::vector<float> t32(2 * n);
+    for (long m = 0; m < n; m++) {
+      double ang = -TWOPI * (double)m / (double)n;
+      t64[2 * m] = cos(ang);
+      t64[2 * m + 1] = sin(ang);
+      t32[2 * m] = (float)t64[2 * m];
+      t32[2 * m + 1] = (float)t64[2 * m + 1];
+    }
+    if (hipMalloc(&c->gd.tw64[a], sizeof(double) * 2 * n) != hipSuccess ||
+        hipMalloc(&c->gd.tw32[a], sizeof(float) * 2 * n) != hipSuccess ||
+        hipMemcpy(c->gd.tw64[a], t64.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(c->gd.tw32[a], t32.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      sep_close(c);
+      return SEP_ERR_HIP;
+    }
+  }
+  c->gd.kxx = tabs_d[0];
+  c->gd.kyy = tabs_d[1];
+  c->gd.kzz = tabs_d[2];
+  c->gd.ekin = c->ekinmax * (ny_1 * nx_1 * nz_1);
+  c->gd.force_lines = getenv("SEP_FORCE_LINE_KERNELS") != nullptr;
+  *out = c;
+  return SEP_OK;
+}
+
+int sep_close(sep_ctx *c) {
+  if (!c) return SEP_OK;
+  hipSetDevice(c->device);
+  for (int a = 0; a < 3; a++) {
+    if (c->gd.tw64[a]) hipFree(c->gd.tw64[a]);
+    if (c->gd.tw32[a]) hipFree(c->gd.tw32[a]);
+  }
+  for (void *p : {(void *)c->gd.kxx, (void *)c->gd.kyy, (void *)c->gd.kzz, (void *)c->pot64,
+                  (void *)c->pot32, (void *)c->states, (void *)c->states2, c->tb, (void *)c->scr,
+                  c->ring})
+    if (p) hipFree(p);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+  return SEP_OK;
+}
+
+// ------------------------------------------------------------- potential
+static int finish_potential(sep_ctx *c) {
+  const long N = c->gd.N;
+  size_t cap = 0;
+  if (!c->pot32) RCHK(ensure(c, (void **)&c->pot32, &cap, sizeof(float) * N));
+  RCHK(convert_d2f(c->pot64, c->pot32, N, c->st));
+  std::vector<double> h(N);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->pot64, sizeof(double) * N, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  double vmin = 1.0e10, vmax = -1.0e10;  // filter/init.c:116-117,196-199
+  for (long g = 0; g < N; g++) {
+    if (vmax < h[g]) vmax = h[g];
+    if (vmin > h[g]) vmin = h[g];
+  }
+  c->vmin = vmin;
+  c->vmax = vmax;
+  c->has_pot = true;
+  return SEP_OK;
+}
+
+int sep_set_potential(sep_ctx *c, const double *potl) {
+  if (!c || !potl) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  size_t cap = 0;
+  if (!c->pot64) RCHK(ensure(c, (void **)&c->pot64, &cap, sizeof(double) * N));
+  HIPCHK(c, hipMemcpyAsync(c->pot64, potl, sizeof(double) * N, hipMemcpyHostToDevice, c->st));
+  return finish_potential(c);
+}
+
+int sep_build_potential(sep_ctx *c, long natom, const double *xyz, const int *type,
+                        const double *scale, int ntype, const long *npot, const double *vr,
+                        const double *pot, long stride, const double *dr, const double *expot,
+                        double *potl_out, double *vmin, double *vmax) {
+  if (!c || natom < 1 || !xyz || !type || ntype < 1 || !npot || !vr || !pot || !dr || stride < 2)
+    return SEP_ERR_ARG;
+  for (long a = 0; a < natom; a++)
+    if (type[a] < 0 || type[a] >= ntype || npot[type[a]] < 2 || npot[type[a]] > stride ||
+        !(dr[type[a]] > 0.0))
+      return fail(c, SEP_ERR_ARG, "atom %ld: bad type or table", a);
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  size_t cap = 0;
+  if (!c->pot64) RCHK(ensure(c, (void **)&c->pot64, &cap, sizeof(double) * N));
+  double *xyz_d, *scale_d = nullptr, *vr_d, *pot_d, *dr_d;
+  int *type_d;
+  long *npot_d;
+  HIPCHK(c, hipMalloc(&xyz_d, sizeof(double) * 3 * natom));
+  HIPCHK(c, hipMalloc(&type_d, sizeof(int) * natom));
+  HIPCHK(c, hipMalloc(&npot_d, sizeof(long) * ntype));
+  HIPCHK(c, hipMalloc(&vr_d, sizeof(double) * ntype * stride));
+  HIPCHK(c, hipMalloc(&pot_d, sizeof(double) * ntype * stride));
+  HIPCHK(c, hipMalloc(&dr_d, sizeof(double) * ntype));
+  if (scale) HIPCHK(c, hipMalloc(&scale_d, sizeof(double) * natom));
+  HIPCHK(c, hipMemcpy(xyz_d, xyz, sizeof(double) * 3 * natom, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(type_d, type, sizeof(int) * natom, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(npot_d, npot, sizeof(long) * ntype, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(vr_d, vr, sizeof(double) * ntype * stride, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(pot_d, pot, sizeof(double) * ntype * stride, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(dr_d, dr, sizeof(double) * ntype, hipMemcpyHostToDevice));
+  if (scale) HIPCHK(c, hipMemcpy(scale_d, scale, sizeof(double) * natom, hipMemcpyHostToDevice));
+  double ex_e = 0.0, ex_x0 = 0.0, ex_t = 1.0;
+  if (expot) {
+    ex_e = expot[0];
+    ex_x0 = expot[1];
+    ex_t = expot[2];
+  }
+  int rc = build_potential_dev(c->gd, c->xmin, c->ymin, c->zmin, c->dx, c->dy, c->dz, natom, xyz_d,
+                               type_d, scale_d, npot_d, vr_d, pot_d, stride, dr_d, ex_e, ex_x0,
+                               ex_t, c->pot64, c->st);
+  hipStreamSynchronize(c->st);
+  hipFree(xyz_d);
+  hipFree(type_d);
+  hipFree(npot_d);
+  hipFree(vr_d);
+  hipFree(pot_d);
+  hipFree(dr_d);
+  if (scale_d) hipFree(scale_d);
+  if (rc != SEP_OK) return fail(c, rc, "potential kernel launch failed");
+  RCHK(finish_potential(c));
+  if (potl_out)
+    HIPCHK(c, hipMemcpy(potl_out, c->pot64, sizeof(double) * N, hipMemcpyDeviceToHost));
+  if (vmin) *vmin = c->vmin;
+  if (vmax) *vmax = c->vmax;
+  return SEP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ H helpers
+template <typename R>
+static const R *pot_of(sep_ctx *c);
+template <>
+const double *pot_of<double>(sep_ctx *c) {
+  return c->pot64;
+}
+template <>
+const float *pot_of<float>(sep_ctx *c) {
+  return c->pot32;
+}
+
+template <typename R>
+static int apply_views(sep_ctx *c, const VecView<R> &in, const VecView<R> &out, int nvec,
+                       double alpha, double beta, std::vector<double> *sums) {
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  const long N = c->gd.N;
+  RCHK(ensure(c, &c->tb, &c->tb_bytes, sizeof(cx<R>) * (size_t)nvec * N));
+  double *part = nullptr;
+  if (sums) {
+    // enough for either kernel shape: rows kernel uses nrows/ROWS blocks per vector
+    size_t need = (size_t)nvec * ((size_t)c->gd.ny * c->gd.nz + c->gd.nz) * 4;
+    RCHK(ensure_scr(c, need));
+    part = c->scr;
+  }
+  long nblk = 0;
+  int rc = hpsi_apply<R>(c->gd, in, out, reinterpret_cast<cx<R> *>(c->tb), pot_of<R>(c), nvec,
+                         alpha, beta, part, &nblk, c->st);
+  if (rc != SEP_OK) return fail(c, rc, "H.psi launch failed (grid %dx%dx%d)", c->gd.nx, c->gd.ny, c->gd.nz);
+  if (sums) {
+    std::vector<double> h((size_t)nblk * 4);
+    HIPCHK(c, hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const long per = nblk / nvec;
+    sums->assign((size_t)nvec * 4, 0.0);
+    for (int v = 0; v < nvec; v++)
+      for (long b = 0; b < per; b++)
+        for (int q = 0; q < 4; q++) (*sums)[v * 4 + q] += h[((size_t)v * per + b) * 4 + q];
+  }
+  return SEP_OK;
+}
+
+// H on m real states stored state-major at src (FP64), two per complex vector.
+// dst may equal src.  Per-state sums: e[i] = sum psi*Hpsi, h2[i] = sum (Hpsi)^2.
+static int apply_states(sep_ctx *c, double *src, double *dst, long m, double *e, double *h2) {
+  const long N = c->gd.N;
+  const long npair = m / 2;
+  std::vector<double> sums;
+  const bool want = e || h2;
+  if (npair > 0) {
+    // bound the FFT scratch: at most 64 complex vectors per launch
+    for (long p0 = 0; p0 < npair; p0 += 64) {
+      long np = npair - p0 < 64 ? npair - p0 : 64;
+      VecView<double> in{src + 2 * p0 * N, src + 2 * p0 * N + N, nullptr, 2 * N};
+      VecView<double> out{dst + 2 * p0 * N, dst + 2 * p0 * N + N, nullptr, 2 * N};
+      RCHK(apply_views<double>(c, in, out, (int)np, 1.0, 0.0, want ? &sums : nullptr));
+      if (want)
+        for (long v = 0; v < np; v++) {
+          if (e) {
+            e[2 * (p0 + v)] = sums[v * 4 + 0];
+            e[2 * (p0 + v) + 1] = sums[v * 4 + 1];
+          }
+          if (h2) {
+            h2[2 * (p0 + v)] = sums[v * 4 + 2];
+            h2[2 * (p0 + v) + 1] = sums[v * 4 + 3];
+          }
+        }
+    }
+  }
+  if (m & 1) {
+    VecView<double> in{src + (m - 1) * N, nullptr, nullptr, N};
+    VecView<double> out{dst + (m - 1) * N, nullptr, nullptr, N};
+    RCHK(apply_views<double>(c, in, out, 1, 1.0, 0.0, want ? &sums : nullptr));
+    if (e) e[m - 1] = sums[0];
+    if (h2) h2[m - 1] = sums[2];
+  }
+  return SEP_OK;
+}
+
+static int normalize_dev(sep_ctx *c, double *x, long m) {
+  const long N = c->gd.N;
+  if (m <= 0) return SEP_OK;
+  const long nb = (N + 16383) / 16384;
+  RCHK(ensure_scr(c, (size_t)nb * m + 2 * m));
+  double *out = c->scr + nb * m;
+  RCHK(dots_dev(x, N, x, N, m, N, out, c->scr, nb * m, c->st));
+  std::vector<double> nrm(m);
+  HIPCHK(c, hipMemcpyAsync(nrm.data(), out, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (long i = 0; i < m; i++) nrm[i] = nrm[i] != 0.0 ? 1.0 / sqrt(nrm[i] * c->dv) : 0.0;  // norm.c:50
+  HIPCHK(c, hipMemcpyAsync(out + m, nrm.data(), sizeof(double) * m, hipMemcpyHostToDevice, c->st));
+  RCHK(scale_rows_dev(x, N, m, N, out + m, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return SEP_OK;
+}
+
+extern "C" {
+
+int sep_apply_h(sep_ctx *c, const double *in, double *out, long batch) {
+  return sep_hnorm(c, in, out, batch, -2.0, 0.0, 1.0);
+}
+
+int sep_hnorm(sep_ctx *c, const double *in, double *out, long batch, double zm1, double vmin,
+              double dE_1) {
+  if (!c || !in || !out || batch < 1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  size_t bytes = sizeof(double) * 2 * (size_t)N * batch;
+  void *d = nullptr;
+  HIPCHK(c, hipMalloc(&d, bytes));
+  int rc = SEP_OK;
+  if (hipMemcpy(d, in, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = SEP_ERR_HIP;
+  // alpha (T + V) psi - beta psi; hamiltonian(): zm1 = -2, vmin = 0, dE_1 = 1
+  const double alpha = dE_1, beta = 2.0 + zm1 + vmin * dE_1;
+  for (long b0 = 0; rc == SEP_OK && b0 < batch; b0 += 64) {
+    long nb = batch - b0 < 64 ? batch - b0 : 64;
+    cx<double> *p = reinterpret_cast<cx<double> *>(d) + b0 * N;
+    VecView<double> v{nullptr, nullptr, p, N};
+    rc = apply_views<double>(c, v, v, (int)nb, alpha, beta, nullptr);
+  }
+  if (rc == SEP_OK && hipMemcpy(out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = SEP_ERR_HIP;
+  hipFree(d);
+  if (rc == SEP_ERR_HIP) return fail(c, rc, "hnorm: HIP copy failed");
+  return rc;
+}
+
+int sep_apply_h_dev(sep_ctx *c, const void *in_dev, void *out_dev, long batch, int precision) {
+  if (!c || !in_dev || !out_dev || batch < 1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  for (long b0 = 0; b0 < batch; b0 += 64) {
+    long nb = batch - b0 < 64 ? batch - b0 : 64;
+    if (precision == SEP_PREC_FP64) {
+      VecView<double> vi{nullptr, nullptr, (cx<double> *)in_dev + b0 * N, N};
+      VecView<double> vo{nullptr, nullptr, (cx<double> *)out_dev + b0 * N, N};
+      RCHK(apply_views<double>(c, vi, vo, (int)nb, 1.0, 0.0, nullptr));
+    } else {
+      VecView<float> vi{nullptr, nullptr, (cx<float> *)in_dev + b0 * N, N};
+      VecView<float> vo{nullptr, nullptr, (cx<float> *)out_dev + b0 * N, N};
+      RCHK(apply_views<float>(c, vi, vo, (int)nb, 1.0, 0.0, nullptr));
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return SEP_OK;
+}
+
+// ----------------------------------------------------------- power iteration
+int sep_energy_range(sep_ctx *c, double vmin, double *dE, double *emax, long *iters) {
+  if (!c) return SEP_ERR_ARG;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  // fixed-seed start, filter/energy.c:57,61
+  std::vector<double> re(N);
+  sep_rng st;
+  sep_rng_reset(&st);
+  long idum = -874917403L;
+  sep_init_psi(&st, &idum, N, c->dv, re.data());
+  std::vector<double> h(2 * N, 0.0);
+  for (long g = 0; g < N; g++) h[2 * g] = re[g];
+  double *v = nullptr, *w = nullptr;
+  HIPCHK(c, hipMalloc(&v, sizeof(double) * 2 * N));
+  HIPCHK(c, hipMalloc(&w, sizeof(double) * 2 * N));
+  HIPCHK(c, hipMemcpy(v, h.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice));
+  VecView<double> V{nullptr, nullptr, (cx<double> *)v, N}, W{nullptr, nullptr, (cx<double> *)w, N};
+  // w = H v.  Iteration i of the reference: psi <- H psi / |H psi|,
+  // ene = <psi|H psi> dv.  H psi for the next iteration is the H psi of
+  // the energy evaluation, so one application per iteration suffices.
+  std::vector<double> sums;
+  int rc = apply_views<double>(c, V, W, 1, 1.0, 0.0, &sums);
+  double ene = 0.0 + 0.1, ene_old = 0.0;
+  long i = 0;
+  const long nb = (2 * N + 16383) / 16384;
+  for (; rc == SEP_OK && (fabs((ene - ene_old) / ene) > 1.0e-4) && (i < 500); i++) {
+    // |w|^2 over the complex vector = sums[2] + sums[3]
+    double norma = sqrt((sums[2] + sums[3]) * c->dv);
+    double inv = 1.0 / norma;
+    if ((rc = ensure_scr(c, nb + 1)) != SEP_OK) break;
+    if (hipMemcpy(c->scr + nb, &inv, sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = SEP_ERR_HIP;
+      break;
+    }
+    // v = w / norma (the normalised H psi)
+    if (hipMemcpyAsync(v, w, sizeof(double) * 2 * N, hipMemcpyDeviceToDevice, c->st) != hipSuccess) {
+      rc = SEP_ERR_HIP;
+      break;
+    }
+    if ((rc = scale_rows_dev(v, 2 * N, 1, 2 * N, c->scr + nb, c->st)) != SEP_OK) break;
+    ene_old = ene;
+    if ((rc = apply_views<double>(c, V, W, 1, 1.0, 0.0, &sums)) != SEP_OK) break;
+    ene = (sums[0] + sums[1]) * c->dv;
+  }
+  hipFree(v);
+  hipFree(w);
+  if (rc != SEP_OK) return rc;
+  if (dE) *dE = 1.1 * (ene - vmin);
+  if (emax) *emax = ene;
+  if (iters) *iters = i;
+  return SEP_OK;
+}
+
+// -------------------------------------------------------------- states
+int sep_states_upload(sep_ctx *c, const double *psi, long m) {
+  if (!c || !psi || m < 1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  RCHK(ensure_states(c, m, 1));
+  HIPCHK(c, hipMemcpy(c->states, psi, sizeof(double) * m * c->gd.N, hipMemcpyHostToDevice));
+  return SEP_OK;
+}
+int sep_states_download(sep_ctx *c, double *psi, long m) {
+  if (!c || !psi || m < 1 || m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(psi, c->states, sizeof(double) * m * c->gd.N, hipMemcpyDeviceToHost));
+  return SEP_OK;
+}
+int sep_normalize_all(sep_ctx *c, long m) {
+  if (!c || m < 0 || m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  return normalize_dev(c, c->states, m);
+}
+int sep_energy_all(sep_ctx *c, long m, double *ene) {
+  if (!c || !ene || m < 1 || m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  RCHK(ensure_states(c, m, 2));
+  RCHK(apply_states(c, c->states, c->states2, m, ene, nullptr));
+  for (long i = 0; i < m; i++) ene[i] *= c->dv;
+  return SEP_OK;
+}
+
+int sep_sigma(sep_ctx *c, long m, double *sige) {
+  if (!c || !sige || m < 1 || m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  RCHK(ensure_states(c, m, 2));
+  std::vector<double> e(m), h2(m);
+  RCHK(apply_states(c, c->states, c->states2, m, e.data(), h2.data()));
+  // <psi|H^2 psi> = |H psi|^2 for symmetric H (one application instead of two)
+  for (long i = 0; i < m; i++) {
+    double ev = e[i] * c->dv;
+    double s2 = h2[i] * c->dv - ev * ev;
+    sige[i] = sqrt(fabs(s2));
+  }
+  return SEP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- filter
+template <typename R>
+static int filter_impl(sep_ctx *c, const double *psi0, bool psi0_on_device, long ns,
+                       const double *an, const double *zn, long nc, long ms, double vmin,
+                       double dE) {
+  const long N = c->gd.N;
+  const double dE_1 = 4.0 / dE;
+  const long nvec = (ns + 1) / 2, S2 = 2 * nvec;
+  // ring of K Newton vectors: K * S2 * N reals
+  long K = nc < 64 ? nc : 64;
+  while (K > 8 && (double)K * S2 * N * sizeof(R) > 64.0e9) K /= 2;
+  if ((long)ms * K * 8 > 64 * 1024) K = (64 * 1024) / (8 * ms);
+  if (K < 1) return fail(c, SEP_ERR_ARG, "ms too large for the accumulation kernel");
+  const long slot = S2 * N;
+  RCHK(ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * slot));
+  R *ring = reinterpret_cast<R *>(c->ring);
+  RCHK(ensure_states(c, ns * ms, 1));
+  HIPCHK(c, hipMemsetAsync(c->states, 0, sizeof(double) * ns * ms * N, c->st));
+  // slot 0 <- psi_0 (padding state zero)
+  HIPCHK(c, hipMemsetAsync(ring, 0, sizeof(R) * slot, c->st));
+  const hipMemcpyKind kind = psi0_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (sizeof(R) == 8) {
+    HIPCHK(c, hipMemcpyAsync(ring, psi0, sizeof(double) * ns * N, kind, c->st));
+  } else if (psi0_on_device) {
+    RCHK(convert_d2f(psi0, (float *)ring, ns * N, c->st));
+  } else {
+    RCHK(ensure_states(c, ns, 2));
+    HIPCHK(c, hipMemcpyAsync(c->states2, psi0, sizeof(double) * ns * N, kind, c->st));
+    RCHK(convert_d2f(c->states2, (float *)ring, ns * N, c->st));
+  }
+  double *an_d = nullptr;
+  HIPCHK(c, hipMalloc(&an_d, sizeof(double) * ms * nc));
+  HIPCHK(c, hipMemcpyAsync(an_d, an, sizeof(double) * ms * nc, hipMemcpyHostToDevice, c->st));
+  int rc = SEP_OK;
+  for (long j = 1; j < nc && rc == SEP_OK; j++) {
+    const long sp = (j - 1) % K, sj = j % K;
+    VecView<R> in{ring + sp * slot, ring + sp * slot + N, nullptr, 2 * N};
+    VecView<R> out{ring + sj * slot, ring + sj * slot + N, nullptr, 2 * N};
+    // hnorm.c:19-24: psi_j = dE_1 H psi_{j-1} - (2 + z_{j-1} + Vmin dE_1) psi_{j-1}
+    const double beta = 2.0 + zn[j - 1] + vmin * dE_1;
+    rc = apply_views<R>(c, in, out, (int)nvec, dE_1, beta, nullptr);
+    if (rc == SEP_OK && (sj == K - 1 || j == nc - 1))
+      rc = accum_dev<R>(ring, slot, (int)(sj + 1), an_d, nc, j - sj, (int)ms, c->states, (int)ns,
+                        N, c->st);
+  }
+  if (rc == SEP_OK && nc == 1) rc = accum_dev<R>(ring, slot, 1, an_d, nc, 0, (int)ms, c->states, (int)ns, N, c->st);
+  hipStreamSynchronize(c->st);
+  hipFree(an_d);
+  if (rc != SEP_OK) return rc;
+  return normalize_dev(c, c->states, ns * ms);  // filter.c:33
+}
+
+extern "C" int sep_filter(sep_ctx *c, const double *psi0, long ns, const double *an,
+                          const double *zn, long nc, long ms, double vmin, double dE,
+                          int precision, double *psitot, double *ene) {
+  if (!c || !psi0 || ns < 1 || !an || !zn || nc < 1 || ms < 1 || !(dE > 0.0)) return SEP_ERR_ARG;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc;
+  if (precision == SEP_PREC_FP64)
+    rc = filter_impl<double>(c, psi0, false, ns, an, zn, nc, ms, vmin, dE);
+  else if (precision == SEP_PREC_FP32)
+    rc = filter_impl<float>(c, psi0, false, ns, an, zn, nc, ms, vmin, dE);
+  else
+    return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
+  if (rc != SEP_OK) return rc;
+  if (ene) RCHK(sep_energy_all(c, ns * ms, ene));
+  if (psitot) RCHK(sep_states_download(c, psitot, ns * ms));
+  return SEP_OK;
+}
+
+// --------------------------------------------------------------- ortho
+extern "C" int sep_ortho(sep_ctx *c, long m, double cutoff, double *S, long *kept) {
+  if (!c || m < 1 || m > c->cap1 || !kept) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  // 1. G = A A^T in double-double (resolves S_i/S_0 well below 1e-10)
+  const long wl = gram_dd_work_len(m, N);
+  RCHK(ensure_scr(c, wl + 2 * m * m));
+  double *ghi_d = c->scr + wl, *glo_d = ghi_d + m * m;
+  RCHK(gram_dd_dev(c->states, m, N, ghi_d, glo_d, c->scr, wl, c->st));
+  std::vector<double> ghi(m * m), glo(m * m);
+  HIPCHK(c, hipMemcpyAsync(ghi.data(), ghi_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(glo.data(), glo_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  // 2. R = chol(G) in double-double; a shift at the double-double rounding
+  //    level keeps exactly dependent states factorable
+  double tr = 0.0;
+  for (long i = 0; i < m; i++) tr += ghi[i * m + i];
+  if (!(tr > 0.0)) return fail(c, SEP_ERR_NUMERIC, "ortho: all states are zero");
+  std::vector<double> R(m * m);
+  // the shift sits above the double-double rounding of G (~1e-27 tr) and is
+  // removed again from the singular values below: (R^T R = G + shift I)
+  double shift = 1.0e-24 * tr;
+  int st = sep_la_chol_dd(m, ghi.data(), glo.data(), shift, R.data());
+  for (int tries = 0; st && tries < 4; tries++) {
+    shift *= 100.0;
+    st = sep_la_chol_dd(m, ghi.data(), glo.data(), shift, R.data());
+  }
+  if (st) return fail(c, SEP_ERR_NUMERIC, "ortho: Cholesky pivot %d not positive", st);
+  // 3. SVD of R: A = (A V S^-1) S V^T
+  std::vector<double> U(m * m), V(m * m), s(m);
+  sep_la_svd_jacobi(m, R.data(), U.data(), s.data(), V.data());
+  for (long i = 0; i < m; i++) {
+    double v = s[i] * s[i] - shift;
+    s[i] = v > 0.0 ? sqrt(v) : 0.0;
+  }
+  long k = m;
+  for (long i = 0; i < m; i++)
+    if (s[i] / s[0] < cutoff) {  // filter/ortho.c:41-46
+      k = i;
+      break;
+    }
+  if (S) memcpy(S, s.data(), sizeof(double) * m);
+  // 4. U_A = A (V_k S_k^-1), then two Cholesky-QR passes for orthonormality
+  std::vector<double> M(m * k);
+  for (long i = 0; i < m; i++)
+    for (long j = 0; j < k; j++) M[i * k + j] = V[i * m + j] / (s[j] > 0.0 ? s[j] : 1.0);
+  RCHK(ensure_states(c, m, 2));
+  RCHK(ensure_scr(c, wl + 2 * m * m));
+  double *M_d = c->scr + wl;
+  HIPCHK(c, hipMemcpyAsync(M_d, M.data(), sizeof(double) * m * k, hipMemcpyHostToDevice, c->st));
+  RCHK(combine_dev<double>(c->states, m, M_d, k, k, N, c->states2, c->st));
+  std::swap(c->states, c->states2);
+  std::swap(c->cap1, c->cap2);
+  for (int pass = 0; pass < 2; pass++) {
+    const long gl = gram_work_len(k, k, N);
+    RCHK(ensure_scr(c, gl + k * k));
+    double *G_d = c->scr + gl;
+    RCHK(gram_dev<double>(c->states, k, c->states, k, N, G_d, c->scr, gl, c->st));
+    std::vector<double> G(k * k), Ri(k * k);
+    HIPCHK(c, hipMemcpyAsync(G.data(), G_d, sizeof(double) * k * k, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    int cs = sep_la_chol_upper(k, G.data());
+    if (cs) return fail(c, SEP_ERR_NUMERIC, "ortho: re-orthogonalisation Cholesky failed (%d)", cs);
+    sep_la_trinv_upper(k, G.data(), Ri.data());
+    HIPCHK(c, hipMemcpyAsync(G_d, Ri.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, c->st));
+    RCHK(combine_dev<double>(c->states, k, G_d, k, k, N, c->states2, c->st));
+    std::swap(c->states, c->states2);
+    std::swap(c->cap1, c->cap2);
+  }
+  *kept = k;
+  return normalize_dev(c, c->states, k);  // filter/main.c:112
+}
+
+// --------------------------------------------------------- Rayleigh-Ritz
+extern "C" int sep_rayleigh_ritz(sep_ctx *c, long m, double *eval) {
+  if (!c || !eval || m < 1 || m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  RCHK(ensure_states(c, m, 2));
+  RCHK(apply_states(c, c->states, c->states2, m, nullptr, nullptr));  // H psi_i
+  const long gl = gram_work_len(m, m, N);
+  RCHK(ensure_scr(c, gl + m * m));
+  double *P_d = c->scr + gl;
+  RCHK(gram_dev<double>(c->states, m, c->states2, m, N, P_d, c->scr, gl, c->st));
+  std::vector<double> P(m * m);
+  HIPCHK(c, hipMemcpyAsync(P.data(), P_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (long i = 0; i < m; i++)
+    for (long j = 0; j < i; j++) {
+      double a = 0.5 * (P[i * m + j] + P[j * m + i]) * c->dv;
+      P[i * m + j] = P[j * m + i] = a;
+    }
+  for (long i = 0; i < m; i++) P[i * m + i] *= c->dv;
+  if (sep_la_symeig(m, P.data(), eval)) return fail(c, SEP_ERR_NUMERIC, "Rayleigh-Ritz: QL failed");
+  HIPCHK(c, hipMemcpyAsync(P_d, P.data(), sizeof(double) * m * m, hipMemcpyHostToDevice, c->st));
+  RCHK(combine_dev<double>(c->states, m, P_d, m, m, N, c->states2, c->st));
+  std::swap(c->states, c->states2);
+  std::swap(c->cap1, c->cap2);
+  return normalize_dev(c, c->states, m);  // filter/main.c:122
+}
+
+extern "C" int sep_filter_dev(sep_ctx *c, const void *psi0_dev, long ns, const double *an,
+                              const double *zn, long nc, long ms, double vmin, double dE,
+                              int precision) {
+  if (!c || !psi0_dev || ns < 1 || !an || !zn || nc < 1 || ms < 1 || !(dE > 0.0))
+    return SEP_ERR_ARG;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  const double *p = static_cast<const double *>(psi0_dev);
+  if (precision == SEP_PREC_FP64) return filter_impl<double>(c, p, true, ns, an, zn, nc, ms, vmin, dE);
+  if (precision == SEP_PREC_FP32) return filter_impl<float>(c, p, true, ns, an, zn, nc, ms, vmin, dE);
+  return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
+}
+
+// ------------------------------------------------------------ measurement
+__global__ void k_fill_random(double *x, long n, unsigned long seed) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    unsigned long z = (unsigned long)i * 0x9E3779B97F4A7C15UL + seed;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9UL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBUL;
+    z ^= z >> 31;
+    x[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+  }
+}
+
+template <typename R>
+static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
+  const long N = c->gd.N;
+  const long S2 = 2 * nvec, K = 64;
+  // ring of K slots of 2*nvec real states (the filter's layout) + accumulators
+  RCHK(ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * S2 * N));
+  RCHK(ensure_states(c, S2 * ms, 1));
+  RCHK(ensure(c, &c->tb, &c->tb_bytes, sizeof(cx<R>) * (size_t)nvec * N));
+  R *ring = reinterpret_cast<R *>(c->ring);
+  if (sizeof(R) == 8) {
+    hipLaunchKernelGGL(k_fill_random, dim3(4096), dim3(256), 0, c->st, (double *)ring, 2 * S2 * N, 7UL);
+  } else {
+    hipLaunchKernelGGL(k_fill_random, dim3(4096), dim3(256), 0, c->st, c->states, S2 * N, 7UL);
+    RCHK(convert_d2f(c->states, (float *)ring, S2 * N, c->st));
+    RCHK(convert_d2f(c->states, (float *)ring + S2 * N, S2 * N, c->st));
+  }
+  HIPCHK(c, hipMemsetAsync(c->states, 0, sizeof(double) * S2 * ms * N, c->st));
+  std::vector<double> an(ms * K);
+  for (long i = 0; i < ms * K; i++) an[i] = 1e-3 * (double)((i * 37) % 101) / 101.0;
+  double *an_d = nullptr;
+  HIPCHK(c, hipMalloc(&an_d, sizeof(double) * ms * K));
+  HIPCHK(c, hipMemcpy(an_d, an.data(), sizeof(double) * ms * K, hipMemcpyHostToDevice));
+  VecView<R> in{ring, ring + N, nullptr, 2 * N};
+  VecView<R> out{ring + S2 * N, ring + S2 * N + N, nullptr, 2 * N};
+  cx<R> *tb = reinterpret_cast<cx<R> *>(c->tb);
+  hipEvent_t ev[6];
+  for (int i = 0; i < 6; i++) HIPCHK(c, hipEventCreate(&ev[i]));
+  double acc[5] = {0, 0, 0, 0, 0};
+  int rc = SEP_OK;
+  for (long it = -1; it < iters && rc == SEP_OK; it++) {  // it = -1: warm-up
+    rc = hpsi_apply_timed<R>(c->gd, in, out, tb, pot_of<R>(c), (int)nvec, 0.3, 1.7, c->st, ev);
+    if (rc == SEP_OK)
+      rc = accum_dev<R>(ring, S2 * N, (int)K, an_d, K, 0, (int)ms, c->states, (int)S2, N, c->st);
+    hipEventRecord(ev[4], c->st);
+    if (hipEventSynchronize(ev[4]) != hipSuccess) rc = SEP_ERR_HIP;
+    if (it >= 0 && rc == SEP_OK) {
+      float t;
+      for (int k = 0; k < 4; k++) {
+        hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+        acc[k] += t;
+      }
+      hipEventElapsedTime(&t, ev[0], ev[3]);
+      acc[4] += t;
+    }
+  }
+  for (int i = 0; i < 6; i++) hipEventDestroy(ev[i]);
+  hipFree(an_d);
+  if (rc != SEP_OK) return fail(c, rc, "bench launch failed");
+  for (int k = 0; k < 5; k++) kms[k] = acc[k] / (double)(iters > 0 ? iters : 1);
+  return SEP_OK;
+}
+
+extern "C" int sep_bench_hpsi(sep_ctx *c, int precision, long nvec, long ms, long iters,
+                              double *kernel_ms) {
+  if (!c || nvec < 1 || ms < 1 || iters < 1 || !kernel_ms) return SEP_ERR_ARG;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (precision == SEP_PREC_FP64) return bench_impl<double>(c, nvec, ms, iters, kernel_ms);
+  if (precision == SEP_PREC_FP32) return bench_impl<float>(c, nvec, ms, iters, kernel_ms);
+  return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
+}
+
+// ------------------------------------------------- dense device contractions
+extern "C" int sep_gram_dev(sep_ctx *c, int precision, const void *A, long m, const void *B,
+                            long mb, double *C) {
+  if (!c || !A || !B || !C || m < 1 || mb < 1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  const long gl = gram_work_len(m, mb, N);
+  RCHK(ensure_scr(c, gl + m * mb));
+  double *C_d = c->scr + gl;
+  if (precision == SEP_PREC_FP64)
+    RCHK(gram_dev<double>((const double *)A, m, (const double *)B, mb, N, C_d, c->scr, gl, c->st));
+  else
+    RCHK(gram_dev<float>((const float *)A, m, (const float *)B, mb, N, C_d, c->scr, gl, c->st));
+  HIPCHK(c, hipMemcpyAsync(C, C_d, sizeof(double) * m * mb, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return SEP_OK;
+}
+
+extern "C" int sep_rotate_dev(sep_ctx *c, int precision, const void *A, long m, const double *M,
+                              long mo, void *Out) {
+  if (!c || !A || !M || !Out || m < 1 || mo < 1 || A == Out) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  RCHK(ensure_scr(c, m * mo));
+  HIPCHK(c, hipMemcpyAsync(c->scr, M, sizeof(double) * m * mo, hipMemcpyHostToDevice, c->st));
+  if (precision == SEP_PREC_FP64)
+    RCHK(combine_dev<double>((const double *)A, m, c->scr, mo, mo, c->gd.N, (double *)Out, c->st));
+  else
+    RCHK(combine_dev<float>((const float *)A, m, c->scr, mo, mo, c->gd.N, (float *)Out, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return SEP_OK;
+}

@@ -1,0 +1,662 @@
+// sep_hpsi.hip -- the H.psi operator of the reference (filter/hamiltonian.c:11-44,
+// filter/hnorm.c:11-39) as hand-written CDNA4 kernels.
+//
+// One application on a batch of complex vectors v (in the filter each complex
+// vector carries two real states, re and im, because H is real) is:
+//   pass A  xy-forward FFT of every z-plane              psi  -> Tbuf
+//   pass B  z-forward FFT, * min(k^2/2, Ekin)/N, z-back   Tbuf -> Tbuf (in place)
+//   pass C  xy-backward FFT of every z-plane, then the fused epilogue
+//           out = alpha (T psi + V psi) - beta psi      Tbuf, psi, V -> out
+// alpha = 1, beta = 0 is hamiltonian(); alpha = dE_1 = 4/dE,
+// beta = 2 + z_{j-1} + Vmin dE_1 is the Newton step of hnorm().
+// Passes A and C keep a whole plane in LDS when it fits (160 KB on gfx950);
+// otherwise each is split into a row kernel and a strided-line kernel.
+#include <hip/hip_runtime.h>
+
+#include "fft_core.h"
+#include "sep_internal.h"
+
+namespace sep {
+
+template <typename R>
+struct Epi {
+  const R *pot;   // V on the grid
+  double alpha, beta;
+  double *part;   // per-block partial sums (4 per block) or nullptr
+};
+
+// block reduction of 4 doubles; writes part[blk*4 + q]
+template <int T>
+__device__ __forceinline__ void block_sum4(double s[4], double *part, long blk) {
+  __shared__ double red[4][T / 64];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    double x = s[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double x = 0.0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++) x += red[threadIdx.x][w];
+    part[blk * 4 + threadIdx.x] = x;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ cx<R> epilogue(const Epi<R> &e, cx<R> t, cx<R> p, long gi, double s[4]) {
+  R V = e.pot[gi];
+  R a = (R)e.alpha, b = (R)e.beta;
+  cx<R> o;
+  o.re = a * (t.re + V * p.re) - b * p.re;
+  o.im = a * (t.im + V * p.im) - b * p.im;
+  s[0] += (double)p.re * (double)o.re;
+  s[1] += (double)p.im * (double)o.im;
+  s[2] += (double)o.re * (double)o.re;
+  s[3] += (double)o.im * (double)o.im;
+  return o;
+}
+
+template <typename R, int N>
+__device__ __forceinline__ void load_tw(cx<R> *dst, const cx<R> *src) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+}
+
+// ======================================================= row kernels (x lines)
+// forward: view -> Tbuf (interleaved)
+template <typename R, int NX, int T>
+__global__ void __launch_bounds__(T) k_rows_fwd(VecView<R> in, cx<R> *tb, long nrows, long N,
+                                                const cx<R> *twx) {
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, ROWS = T / G, P = PL::P;
+  __shared__ cx<R> buf[ROWS][NX];
+  __shared__ cx<R> tw[NX];
+  load_tw<R, NX>(tw, twx);
+  __syncthreads();
+  const long vec = blockIdx.y;
+  const int rl = threadIdx.x / G;
+  const int g = threadIdx.x % G;
+  const long row = (long)blockIdx.x * ROWS + rl;
+  const bool act = (rl < ROWS) && (row < nrows);
+  const int rls = rl < ROWS ? rl : 0;
+  cx<R> v[E];
+  constexpr int r0 = Pass<NX, -1>::radix(0);
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / r0; b++)
+#pragma unroll
+      for (int k = 0; k < r0; k++) v[b * r0 + k] = in.ld(vec, row * NX + in_idx<NX, -1, 0>(g, b, k));
+  }
+  run_passes<NX, -1, 0, P>(v, g, act, tw, [&](int i) -> cx<R> & { return buf[rls][i]; });
+  constexpr int rl_ = Pass<NX, -1>::radix(P - 1);
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+      for (int k = 0; k < rl_; k++)
+        tb[vec * N + row * NX + out_idx<NX, -1, P - 1>(g, b, k)] = v[b * rl_ + k];
+  }
+}
+
+// backward + epilogue: Tbuf -> view (psi_in read from `pin`)
+template <typename R, int NX, int T>
+__global__ void __launch_bounds__(T) k_rows_bwd(const cx<R> *tb, VecView<R> pin, VecView<R> out,
+                                                Epi<R> epi, long nrows, long N, const cx<R> *twx) {
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, ROWS = T / G, P = PL::P;
+  __shared__ cx<R> buf[ROWS][NX];
+  __shared__ cx<R> tw[NX];
+  load_tw<R, NX>(tw, twx);
+  __syncthreads();
+  const long vec = blockIdx.y;
+  const int rl = threadIdx.x / G;
+  const int g = threadIdx.x % G;
+  const long row = (long)blockIdx.x * ROWS + rl;
+  const bool act = (rl < ROWS) && (row < nrows);
+  const int rls = rl < ROWS ? rl : 0;
+  cx<R> v[E];
+  constexpr int r0 = Pass<NX, 1>::radix(0);
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / r0; b++)
+#pragma unroll
+      for (int k = 0; k < r0; k++) v[b * r0 + k] = tb[vec * N + row * NX + in_idx<NX, 1, 0>(g, b, k)];
+  }
+  run_passes<NX, 1, 0, P>(v, g, act, tw, [&](int i) -> cx<R> & { return buf[rls][i]; });
+  constexpr int rl_ = Pass<NX, 1>::radix(P - 1);
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+      for (int k = 0; k < rl_; k++) {
+        long gi = row * NX + out_idx<NX, 1, P - 1>(g, b, k);
+        cx<R> p = pin.ld(vec, gi);
+        out.st(vec, gi, epilogue(epi, v[b * rl_ + k], p, gi, s));
+      }
+  }
+  if (epi.part) block_sum4<T>(s, epi.part, (long)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// =============================================== strided lines (y, fallback)
+// Lines of length N along an axis with element stride `es`; the block takes W
+// consecutive x positions.  Line base for (x, outer) = outer*os + x.
+template <typename R, int NL, int W, int S>
+__global__ void __launch_bounds__(W * FftPlan<NL>::G) k_lines(cx<R> *tb, long N, int nx, long es,
+                                                              long os, const cx<R> *twl) {
+  using PL = FftPlan<NL>;
+  constexpr int G = PL::G, E = PL::E, P = PL::P;
+  __shared__ cx<R> buf[NL][W];
+  __shared__ cx<R> tw[NL];
+  load_tw<R, NL>(tw, twl);
+  __syncthreads();
+  const int p = threadIdx.x % W;
+  const int g = threadIdx.x / W;
+  const long x = (long)blockIdx.x * W + p;
+  const long outer = blockIdx.y;
+  const long vec = blockIdx.z;
+  const bool act = x < nx;
+  cx<R> *base = tb + vec * N + outer * os + x;
+  cx<R> v[E];
+  constexpr int r0 = Pass<NL, S>::radix(0);
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / r0; b++)
+#pragma unroll
+      for (int k = 0; k < r0; k++) v[b * r0 + k] = base[(long)in_idx<NL, S, 0>(g, b, k) * es];
+  }
+  run_passes<NL, S, 0, P>(v, g, act, tw, [&](int i) -> cx<R> & { return buf[i][p]; });
+  constexpr int rl_ = Pass<NL, S>::radix(P - 1);
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+      for (int k = 0; k < rl_; k++) base[(long)out_idx<NL, S, P - 1>(g, b, k) * es] = v[b * rl_ + k];
+  }
+}
+
+// ======================================== z pencils: forward, * k^2, backward
+template <typename R, int NZ, int W>
+__global__ void __launch_bounds__(W * FftPlan<NZ>::G) k_pencil(cx<R> *tb, long N, int nx, int ny,
+                                                               const cx<R> *twz,
+                                                               const double *kxx,
+                                                               const double *kyy,
+                                                               const double *kzz, double ekin,
+                                                               double kscale) {
+  using PL = FftPlan<NZ>;
+  constexpr int G = PL::G, E = PL::E, P = PL::P;
+  __shared__ cx<R> buf[NZ][W];
+  __shared__ cx<R> tw[NZ];
+  __shared__ double kz_s[NZ];
+  load_tw<R, NZ>(tw, twz);
+  for (int i = threadIdx.x; i < NZ; i += blockDim.x) kz_s[i] = kzz[i];
+  __syncthreads();
+  const int p = threadIdx.x % W;
+  const int g = threadIdx.x / W;
+  const long x = (long)blockIdx.x * W + p;
+  const long y = blockIdx.y;
+  const long vec = blockIdx.z;
+  const bool act = x < nx;
+  const long es = (long)nx * ny;
+  cx<R> *base = tb + vec * N + y * nx + x;
+  cx<R> v[E];
+  constexpr int r0 = Pass<NZ, -1>::radix(0);
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / r0; b++)
+#pragma unroll
+      for (int k = 0; k < r0; k++) v[b * r0 + k] = base[(long)in_idx<NZ, -1, 0>(g, b, k) * es];
+  }
+  auto at = [&](int i) -> cx<R> & { return buf[i][p]; };
+  run_passes<NZ, -1, 0, P>(v, g, act, tw, at);
+  // kinetic factor, filter/init.c:138-144 order: (kx + ky) + kz, clamp
+  const double kxy = act ? (kxx[x] + kyy[y]) : 0.0;
+  constexpr int rl_ = Pass<NZ, -1>::radix(P - 1);
+#pragma unroll
+  for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+    for (int k = 0; k < rl_; k++) {
+      int z = out_idx<NZ, -1, P - 1>(g, b, k);
+      double kin = kxy + kz_s[z];
+      if (kin > ekin) kin = ekin;
+      v[b * rl_ + k] = cscale(v[b * rl_ + k], (R)(kin * kscale));
+    }
+  // backward: pass 0 reads exactly the registers the forward last pass wrote
+  run_passes<NZ, 1, 0, P>(v, g, act, tw, at);
+  constexpr int rb = Pass<NZ, 1>::radix(P - 1);
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / rb; b++)
+#pragma unroll
+      for (int k = 0; k < rb; k++) base[(long)out_idx<NZ, 1, P - 1>(g, b, k) * es] = v[b * rb + k];
+  }
+}
+
+// ================================================ xy planes held in LDS
+template <int NX>
+struct PlaneCfg {
+  static constexpr int LD = NX + 1;  // padded row pitch
+};
+
+// forward: view plane -> x FFT rows -> LDS -> y FFT columns -> Tbuf
+template <typename R, int NX, int T>
+__global__ void __launch_bounds__(T) k_plane_fwd(VecView<R> in, cx<R> *tb, long N, int nz,
+                                                 const cx<R> *twg) {
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, P = PL::P, LD = PlaneCfg<NX>::LD;
+  constexpr int RX = T / G;  // lines per round
+  extern __shared__ char smem_raw[];
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
+  cx<R> *tw = buf + NX * LD;
+  load_tw<R, NX>(tw, twg);
+  __syncthreads();
+  const long vec = blockIdx.x / nz;
+  const long z = blockIdx.x % nz;
+  const long pbase = z * (long)NX * NX;
+  const int tl = threadIdx.x / G, g = threadIdx.x % G;
+  constexpr int r0 = Pass<NX, -1>::radix(0);
+  constexpr int rl_ = Pass<NX, -1>::radix(P - 1);
+  // ---- rows
+  for (int y0 = 0; y0 < NX; y0 += RX) {
+    const int row = y0 + tl;
+    const bool act = (tl < RX) && (row < NX);
+    const int rs = act ? row : 0;
+    cx<R> v[E];
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / r0; b++)
+#pragma unroll
+        for (int k = 0; k < r0; k++) v[b * r0 + k] = in.ld(vec, pbase + rs * NX + in_idx<NX, -1, 0>(g, b, k));
+    }
+    auto at = [&](int i) -> cx<R> & { return buf[rs * LD + i]; };
+    run_passes<NX, -1, 0, P>(v, g, act, tw, at);
+    if constexpr (P > 1) __syncthreads();
+    write_out<NX, -1, P - 1>(v, g, act, at);
+  }
+  __syncthreads();
+  // ---- columns (consecutive threads -> consecutive columns)
+  const int cl = threadIdx.x % RX, gc = threadIdx.x / RX;
+  for (int x0 = 0; x0 < NX; x0 += RX) {
+    const int col = x0 + cl;
+    const bool act = (gc < G) && (col < NX);
+    const int cs = act ? col : 0;
+    const int gs = act ? gc : 0;
+    cx<R> v[E];
+    auto at = [&](int i) -> cx<R> & { return buf[i * LD + cs]; };
+    read_in<NX, -1, 0>(v, gs, act, at);
+    run_passes<NX, -1, 0, P>(v, gs, act, tw, at);
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+        for (int k = 0; k < rl_; k++)
+          tb[vec * N + pbase + (long)out_idx<NX, -1, P - 1>(gs, b, k) * NX + cs] = v[b * rl_ + k];
+    }
+    __syncthreads();
+  }
+}
+
+// backward + epilogue: Tbuf plane -> y FFT columns -> LDS -> x FFT rows -> out
+template <typename R, int NX, int T>
+__global__ void __launch_bounds__(T) k_plane_bwd(const cx<R> *tb, VecView<R> pin, VecView<R> out,
+                                                 Epi<R> epi, long N, int nz, const cx<R> *twg) {
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, P = PL::P, LD = PlaneCfg<NX>::LD;
+  constexpr int RX = T / G;
+  extern __shared__ char smem_raw[];
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
+  cx<R> *tw = buf + NX * LD;
+  load_tw<R, NX>(tw, twg);
+  __syncthreads();
+  const long vec = blockIdx.x / nz;
+  const long z = blockIdx.x % nz;
+  const long pbase = z * (long)NX * NX;
+  constexpr int r0 = Pass<NX, 1>::radix(0);
+  constexpr int rl_ = Pass<NX, 1>::radix(P - 1);
+  // ---- columns
+  const int cl = threadIdx.x % RX, gc = threadIdx.x / RX;
+  for (int x0 = 0; x0 < NX; x0 += RX) {
+    const int col = x0 + cl;
+    const bool act = (gc < G) && (col < NX);
+    const int cs = act ? col : 0;
+    const int gs = act ? gc : 0;
+    cx<R> v[E];
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / r0; b++)
+#pragma unroll
+        for (int k = 0; k < r0; k++)
+          v[b * r0 + k] = tb[vec * N + pbase + (long)in_idx<NX, 1, 0>(gs, b, k) * NX + cs];
+    }
+    auto at = [&](int i) -> cx<R> & { return buf[i * LD + cs]; };
+    run_passes<NX, 1, 0, P>(v, gs, act, tw, at);
+    if constexpr (P > 1) __syncthreads();
+    write_out<NX, 1, P - 1>(v, gs, act, at);
+  }
+  __syncthreads();
+  // ---- rows + epilogue
+  const int tl = threadIdx.x / G, g = threadIdx.x % G;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int y0 = 0; y0 < NX; y0 += RX) {
+    const int row = y0 + tl;
+    const bool act = (tl < RX) && (row < NX);
+    const int rs = act ? row : 0;
+    cx<R> v[E];
+    auto at = [&](int i) -> cx<R> & { return buf[rs * LD + i]; };
+    read_in<NX, 1, 0>(v, g, act, at);
+    run_passes<NX, 1, 0, P>(v, g, act, tw, at);
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+        for (int k = 0; k < rl_; k++) {
+          long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
+          cx<R> p = pin.ld(vec, gi);
+          out.st(vec, gi, epilogue(epi, v[b * rl_ + k], p, gi, s));
+        }
+    }
+    __syncthreads();
+  }
+  if (epi.part) block_sum4<T>(s, epi.part, blockIdx.x);
+}
+
+// ================================================================= dispatch
+template <typename R, int NX>
+constexpr size_t plane_lds() {
+  return sizeof(cx<R>) * ((size_t)NX * PlaneCfg<NX>::LD + NX);
+}
+template <typename R, int NX>
+constexpr int plane_threads() {
+  return NX >= 64 ? 512 : 256;
+}
+
+template <typename R>
+struct HOps {
+  // returns false if size unsupported
+  static bool rows_fwd(int nx, VecView<R> in, cx<R> *tb, long nrows, long N, int nvec,
+                       const cx<R> *tw, hipStream_t st);
+  static bool rows_bwd(int nx, const cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
+                       long nrows, long N, int nvec, const cx<R> *tw, hipStream_t st,
+                       long *nblk);
+  static bool lines(int nl, int S, cx<R> *tb, long N, int nx, long es, long os, int nouter,
+                    int nvec, const cx<R> *tw, hipStream_t st);
+  static bool pencil(int nz, cx<R> *tb, long N, int nx, int ny, int nvec, const cx<R> *tw,
+                     const double *kxx, const double *kyy, const double *kzz, double ekin,
+                     double kscale, hipStream_t st);
+  static bool plane_fwd(int nxy, VecView<R> in, cx<R> *tb, long N, int nz, int nvec,
+                        const cx<R> *tw, hipStream_t st);
+  static bool plane_bwd(int nxy, const cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
+                        long N, int nz, int nvec, const cx<R> *tw, hipStream_t st, long *nblk);
+};
+
+#define SEP_LINE_SIZES(X) X(16) X(24) X(32) X(36) X(48) X(64) X(80) X(96) X(128) X(160)
+
+template <typename R>
+bool HOps<R>::rows_fwd(int nx, VecView<R> in, cx<R> *tb, long nrows, long N, int nvec,
+                       const cx<R> *tw, hipStream_t st) {
+  switch (nx) {
+#define C(n)                                                                              \
+  case n: {                                                                               \
+    constexpr int T = 256, ROWS = T / FftPlan<n>::G;                                      \
+    dim3 grid((unsigned)((nrows + ROWS - 1) / ROWS), (unsigned)nvec);                     \
+    hipLaunchKernelGGL((k_rows_fwd<R, n, T>), grid, dim3(T), 0, st, in, tb, nrows, N, tw); \
+    return true;                                                                          \
+  }
+    SEP_LINE_SIZES(C)
+#undef C
+  }
+  return false;
+}
+
+template <typename R>
+bool HOps<R>::rows_bwd(int nx, const cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
+                       long nrows, long N, int nvec, const cx<R> *tw, hipStream_t st,
+                       long *nblk) {
+  switch (nx) {
+#define C(n)                                                                                \
+  case n: {                                                                                 \
+    constexpr int T = 256, ROWS = T / FftPlan<n>::G;                                        \
+    dim3 grid((unsigned)((nrows + ROWS - 1) / ROWS), (unsigned)nvec);                       \
+    if (nblk) *nblk = (long)grid.x * grid.y;                                                \
+    hipLaunchKernelGGL((k_rows_bwd<R, n, T>), grid, dim3(T), 0, st, tb, pin, out, epi, nrows, \
+                       N, tw);                                                              \
+    return true;                                                                            \
+  }
+    SEP_LINE_SIZES(C)
+#undef C
+  }
+  return false;
+}
+
+template <typename R>
+bool HOps<R>::lines(int nl, int S, cx<R> *tb, long N, int nx, long es, long os, int nouter,
+                    int nvec, const cx<R> *tw, hipStream_t st) {
+  constexpr int W = sizeof(R) == 8 ? 16 : 32;
+  dim3 grid((unsigned)((nx + W - 1) / W), (unsigned)nouter, (unsigned)nvec);
+  switch (nl) {
+#define C(n)                                                                                   \
+  case n:                                                                                      \
+    if (S < 0)                                                                                 \
+      hipLaunchKernelGGL((k_lines<R, n, W, -1>), grid, dim3(W * FftPlan<n>::G), 0, st, tb, N, nx, \
+                         es, os, tw);                                                          \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_lines<R, n, W, 1>), grid, dim3(W * FftPlan<n>::G), 0, st, tb, N, nx,  \
+                         es, os, tw);                                                          \
+    return true;
+    SEP_LINE_SIZES(C)
+#undef C
+  }
+  return false;
+}
+
+template <typename R>
+bool HOps<R>::pencil(int nz, cx<R> *tb, long N, int nx, int ny, int nvec, const cx<R> *tw,
+                     const double *kxx, const double *kyy, const double *kzz, double ekin,
+                     double kscale, hipStream_t st) {
+  constexpr int W = sizeof(R) == 8 ? 16 : 32;
+  dim3 grid((unsigned)((nx + W - 1) / W), (unsigned)ny, (unsigned)nvec);
+  switch (nz) {
+#define C(n)                                                                                  \
+  case n:                                                                                     \
+    hipLaunchKernelGGL((k_pencil<R, n, W>), grid, dim3(W * FftPlan<n>::G), 0, st, tb, N, nx, ny, \
+                       tw, kxx, kyy, kzz, ekin, kscale);                                      \
+    return true;
+    SEP_LINE_SIZES(C)
+#undef C
+  }
+  return false;
+}
+
+#define SEP_PLANE_SIZES_F32(X) X(16) X(24) X(32) X(36) X(48) X(64) X(80) X(96) X(128)
+#define SEP_PLANE_SIZES_F64(X) X(16) X(24) X(32) X(36) X(48) X(64) X(80) X(96)
+
+template <typename R, int n>
+static void plane_fwd_launch(VecView<R> in, cx<R> *tb, long N, int nz, int nvec, const cx<R> *tw,
+                             hipStream_t st) {
+  constexpr int T = plane_threads<R, n>();
+  constexpr size_t lds = plane_lds<R, n>();
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void *)k_plane_fwd<R, n, T>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_plane_fwd<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, in, tb,
+                     N, nz, tw);
+}
+template <typename R, int n>
+static void plane_bwd_launch(const cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi, long N,
+                             int nz, int nvec, const cx<R> *tw, hipStream_t st) {
+  constexpr int T = plane_threads<R, n>();
+  constexpr size_t lds = plane_lds<R, n>();
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void *)k_plane_bwd<R, n, T>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_plane_bwd<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, tb, pin,
+                     out, epi, N, nz, tw);
+}
+
+template <>
+bool HOps<float>::plane_fwd(int nxy, VecView<float> in, cx<float> *tb, long N, int nz, int nvec,
+                            const cx<float> *tw, hipStream_t st) {
+  switch (nxy) {
+#define C(n)                                                 \
+  case n:                                                    \
+    plane_fwd_launch<float, n>(in, tb, N, nz, nvec, tw, st); \
+    return true;
+    SEP_PLANE_SIZES_F32(C)
+#undef C
+  }
+  return false;
+}
+template <>
+bool HOps<double>::plane_fwd(int nxy, VecView<double> in, cx<double> *tb, long N, int nz,
+                             int nvec, const cx<double> *tw, hipStream_t st) {
+  switch (nxy) {
+#define C(n)                                                  \
+  case n:                                                     \
+    plane_fwd_launch<double, n>(in, tb, N, nz, nvec, tw, st); \
+    return true;
+    SEP_PLANE_SIZES_F64(C)
+#undef C
+  }
+  return false;
+}
+template <>
+bool HOps<float>::plane_bwd(int nxy, const cx<float> *tb, VecView<float> pin, VecView<float> out,
+                            Epi<float> epi, long N, int nz, int nvec, const cx<float> *tw,
+                            hipStream_t st, long *nblk) {
+  if (nblk) *nblk = (long)nz * nvec;
+  switch (nxy) {
+#define C(n)                                                                 \
+  case n:                                                                    \
+    plane_bwd_launch<float, n>(tb, pin, out, epi, N, nz, nvec, tw, st);      \
+    return true;
+    SEP_PLANE_SIZES_F32(C)
+#undef C
+  }
+  return false;
+}
+template <>
+bool HOps<double>::plane_bwd(int nxy, const cx<double> *tb, VecView<double> pin,
+                             VecView<double> out, Epi<double> epi, long N, int nz, int nvec,
+                             const cx<double> *tw, hipStream_t st, long *nblk) {
+  if (nblk) *nblk = (long)nz * nvec;
+  switch (nxy) {
+#define C(n)                                                                 \
+  case n:                                                                    \
+    plane_bwd_launch<double, n>(tb, pin, out, epi, N, nz, nvec, tw, st);     \
+    return true;
+    SEP_PLANE_SIZES_F64(C)
+#undef C
+  }
+  return false;
+}
+
+// ----------------------------------------------------------------- public
+bool hop_supported(int n) {
+  switch (n) {
+#define C(x) case x:
+    SEP_LINE_SIZES(C)
+#undef C
+    return true;
+  }
+  return false;
+}
+
+bool plane_supported(int nx, int ny, int prec) {
+  if (nx != ny) return false;
+  if (prec == SEP_PREC_FP64) {
+    switch (nx) {
+#define C(x) case x:
+      SEP_PLANE_SIZES_F64(C)
+#undef C
+      return true;
+    }
+    return false;
+  }
+  switch (nx) {
+#define C(x) case x:
+    SEP_PLANE_SIZES_F32(C)
+#undef C
+    return true;
+  }
+  return false;
+}
+
+// One H application (see file header) on nvec complex vectors.
+template <typename R>
+int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
+               const R *pot, int nvec, double alpha, double beta, double *part, long *nblk,
+               hipStream_t st) {
+  const int nx = gd.nx, ny = gd.ny, nz = gd.nz;
+  const long N = gd.N;
+  const cx<R> *twx = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
+  const cx<R> *twy = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[1] : gd.tw32[1]);
+  const cx<R> *twz = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[2] : gd.tw32[2]);
+  const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
+  const bool plane = gd.force_lines ? false : plane_supported(nx, ny, prec);
+  // pass A
+  if (plane) {
+    if (!HOps<R>::plane_fwd(nx, in, tb, N, nz, nvec, twx, st)) return SEP_ERR_UNSUPPORTED;
+  } else {
+    if (!HOps<R>::rows_fwd(nx, in, tb, (long)ny * nz, N, nvec, twx, st)) return SEP_ERR_UNSUPPORTED;
+    if (!HOps<R>::lines(ny, -1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st))
+      return SEP_ERR_UNSUPPORTED;
+  }
+  // pass B
+  if (!HOps<R>::pencil(nz, tb, N, nx, ny, nvec, twz, gd.kxx, gd.kyy, gd.kzz, gd.ekin, 1.0, st))
+    return SEP_ERR_UNSUPPORTED;
+  // pass C
+  Epi<R> epi{pot, alpha, beta, part};
+  if (plane) {
+    if (!HOps<R>::plane_bwd(nx, tb, in, out, epi, N, nz, nvec, twx, st, nblk))
+      return SEP_ERR_UNSUPPORTED;
+  } else {
+    if (!HOps<R>::lines(ny, 1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st))
+      return SEP_ERR_UNSUPPORTED;
+    if (!HOps<R>::rows_bwd(nx, tb, in, out, epi, (long)ny * nz, N, nvec, twx, st, nblk))
+      return SEP_ERR_UNSUPPORTED;
+  }
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
+// The three passes of one application with an event recorded before each
+// pass and after the last (ev[0..3]); plane kernels only.
+template <typename R>
+int hpsi_apply_timed(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
+                     const R *pot, int nvec, double alpha, double beta, hipStream_t st,
+                     hipEvent_t *ev) {
+  const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
+  if (gd.force_lines || !plane_supported(gd.nx, gd.ny, prec)) return SEP_ERR_UNSUPPORTED;
+  const cx<R> *twx = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
+  const cx<R> *twz = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[2] : gd.tw32[2]);
+  Epi<R> epi{pot, alpha, beta, nullptr};
+  hipEventRecord(ev[0], st);
+  HOps<R>::plane_fwd(gd.nx, in, tb, gd.N, gd.nz, nvec, twx, st);
+  hipEventRecord(ev[1], st);
+  HOps<R>::pencil(gd.nz, tb, gd.N, gd.nx, gd.ny, nvec, twz, gd.kxx, gd.kyy, gd.kzz, gd.ekin, 1.0, st);
+  hipEventRecord(ev[2], st);
+  HOps<R>::plane_bwd(gd.nx, tb, in, out, epi, gd.N, gd.nz, nvec, twx, st, nullptr);
+  hipEventRecord(ev[3], st);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+template int hpsi_apply_timed<double>(const GridDev &, const VecView<double> &,
+                                      const VecView<double> &, cx<double> *, const double *, int,
+                                      double, double, hipStream_t, hipEvent_t *);
+template int hpsi_apply_timed<float>(const GridDev &, const VecView<float> &,
+                                     const VecView<float> &, cx<float> *, const float *, int,
+                                     double, double, hipStream_t, hipEvent_t *);
+
+template int hpsi_apply<double>(const GridDev &, const VecView<double> &, const VecView<double> &,
+                                cx<double> *, const double *, int, double, double, double *,
+                                long *, hipStream_t);
+template int hpsi_apply<float>(const GridDev &, const VecView<float> &, const VecView<float> &,
+                               cx<float> *, const float *, int, double, double, double *, long *,
+                               hipStream_t);
+
+}  // namespace sep

@@ -1,0 +1,70 @@
+// sep_internal.h -- declarations shared by the libsep translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/sep.h"
+#include "fft_core.h"
+#include "sep_host.h"
+
+namespace sep {
+
+// Device-resident description of one grid.
+struct GridDev {
+  int nx, ny, nz;
+  long N;
+  void *tw64[3];  // exp(-2 pi i m/n) per axis, complex128
+  void *tw32[3];  // same, complex64
+  double *kxx, *kyy, *kzz;  // filter/init.c:125-135 axis tables (1/N folded in)
+  double ekin;              // Ekinmax / N (filter/init.c:137)
+  bool force_lines;         // test hook: use the row/line kernels even when a plane fits
+};
+
+bool hop_supported(int n);
+bool plane_supported(int nx, int ny, int prec);
+
+template <typename R>
+int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
+               const R *pot, int nvec, double alpha, double beta, double *part, long *nblk,
+               hipStream_t st);
+
+template <typename R>
+int hpsi_apply_timed(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
+                     const R *pot, int nvec, double alpha, double beta, hipStream_t st,
+                     hipEvent_t *ev);
+
+// ---- element-wise / reduction kernels (sep_kernels.hip)
+int build_potential_dev(const GridDev &gd, double xmin, double ymin, double zmin, double dx,
+                        double dy, double dz, long natom, const double *xyz_d, const int *type_d,
+                        const double *scale_d, const long *npot_d, const double *vr_d,
+                        const double *pot_d, long stride, const double *dr_d, double ex_e,
+                        double ex_x0, double ex_t, double *potl_d, hipStream_t st);
+// out[i] = sum_g a[i*lda + g] * b[i*ldb + g] for i < m (deterministic).
+int dots_dev(const double *a, long lda, const double *b, long ldb, long m, long n, double *out_d,
+             double *scratch, long scratch_len, hipStream_t st);
+int dots_dev_f(const float *a, long lda, const float *b, long ldb, long m, long n, double *out_d,
+               double *scratch, long scratch_len, hipStream_t st);
+// x[i*ld + g] *= s[i]
+int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStream_t st);
+int convert_d2f(const double *in, float *out, long n, hipStream_t st);
+int convert_f2d(const float *in, double *out, long n, hipStream_t st);
+// acc[(s*ms + ie)*N + g] += sum_{k<cnt} an[ie*nc + j0 + k] * ring[k*slot + s*N + g]
+template <typename R>
+int accum_dev(const R *ring, long slot, int cnt, const double *an_d, long nc, long j0, int ms,
+              double *acc, int ns, long N, hipStream_t st);
+
+// ---- GEMMs (sep_gemm.hip); all operands state-major (row i = one state)
+// C[i][j] = sum_g A[i][g] B[j][g]   (m x mb), deterministic split-K
+template <typename R>
+int gram_dev(const R *A, long m, const R *B, long mb, long N, double *C_d, double *work,
+             long work_len, hipStream_t st);
+// Out[j][g] = sum_i M[i*ldm + j] A[i][g]  (mo states out), Out != A
+template <typename R>
+int combine_dev(const R *A, long m, const double *M_d, long ldm, long mo, long N, R *Out,
+                hipStream_t st);
+long gram_work_len(long m, long mb, long N);
+// G = A A^T as double-double (hi + lo), m x m row-major
+int gram_dd_dev(const double *A, long m, long N, double *hi_d, double *lo_d, double *work,
+                long work_len, hipStream_t st);
+long gram_dd_work_len(long m, long N);
+
+}  // namespace sep

@@ -1,0 +1,212 @@
+// sep_kernels.hip -- non-FFT kernels of libsep: V(r) assembly, deterministic
+// per-state reductions, the K-blocked Newton accumulation and the
+// double-double Gram used to decide the SVD cutoff.
+#include <hip/hip_runtime.h>
+
+#include "fft_core.h"
+#include "sep_internal.h"
+
+namespace sep {
+
+// ------------------------------------------------------------ V(r) assembly
+// filter/init.c:161-182 (sum over atoms in file order, + expot), interpolate
+// of filter/interpolate.c:11-23 with the per-atom scale of
+// filterStrainCube/interpolate.c:20-21.  Contraction is off so each term is
+// rounded exactly as the reference source writes it.
+__global__ void __launch_bounds__(256) k_build_potential(
+    int nx, int ny, int nz, double xmin, double ymin, double zmin, double dx, double dy, double dz,
+    long natom, const double *__restrict__ xyz, const int *__restrict__ type,
+    const double *__restrict__ scale, const long *__restrict__ npot, const double *__restrict__ vr,
+    const double *__restrict__ pot, long stride, const double *__restrict__ dr, double ex_e,
+    double ex_x0, double ex_t, double *__restrict__ potl) {
+#pragma clang fp contract(off)
+  const long N = (long)nx * ny * nz;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double sx[256], sy[256], sz[256], ss[256];
+  __shared__ int st[256];
+  // grid coordinates by repeated addition (filter/init.c:157-159)
+  int jx = 0, jy = 0, jz = 0;
+  double px = xmin, py = ymin, pz = zmin;
+  if (g < N) {
+    jx = (int)(g % nx);
+    jy = (int)((g / nx) % ny);
+    jz = (int)(g / ((long)nx * ny));
+    for (int i = 0; i < jx; i++) px += dx;
+    for (int i = 0; i < jy; i++) py += dy;
+    for (int i = 0; i < jz; i++) pz += dz;
+  }
+  double sum = 0.0;
+  for (long a0 = 0; a0 < natom; a0 += 256) {
+    __syncthreads();
+    long a = a0 + threadIdx.x;
+    if (a < natom) {
+      sx[threadIdx.x] = xyz[3 * a];
+      sy[threadIdx.x] = xyz[3 * a + 1];
+      sz[threadIdx.x] = xyz[3 * a + 2];
+      ss[threadIdx.x] = scale ? scale[a] : 1.0;
+      st[threadIdx.x] = type[a];
+    }
+    __syncthreads();
+    const int na = (int)((natom - a0) < 256 ? (natom - a0) : 256);
+    if (g < N) {
+      for (int k = 0; k < na; k++) {
+        double ddx = px - sx[k], ddy = py - sy[k], ddz = pz - sz[k];
+        double del = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+        const int t = st[k];
+        const double drt = dr[t];
+        long i = (long)(del / drt);
+        if (i > npot[t] - 2) continue;  // adds 0.0 in the reference
+        const double *p = pot + t * stride, *v = vr + t * stride;
+        double aa = ss[k] * (p[i + 1] - p[i]) / (v[i + 1] - v[i]);
+        double bb = ss[k] * p[i] - v[i] * aa;
+        sum += aa * del + bb;
+      }
+    }
+  }
+  if (g < N) {
+    double rex = sqrt(px * px + py * py + pz * pz);
+    double potEx = ex_e * (1.0 / (exp((rex - ex_x0) / ex_t) + 1.0) - 1.0);
+    potl[g] = sum + potEx;
+  }
+}
+
+int build_potential_dev(const GridDev &gd, double xmin, double ymin, double zmin, double dx,
+                        double dy, double dz, long natom, const double *xyz_d, const int *type_d,
+                        const double *scale_d, const long *npot_d, const double *vr_d,
+                        const double *pot_d, long stride, const double *dr_d, double ex_e,
+                        double ex_x0, double ex_t, double *potl_d, hipStream_t st) {
+  dim3 grid((unsigned)((gd.N + 255) / 256));
+  hipLaunchKernelGGL(k_build_potential, grid, dim3(256), 0, st, gd.nx, gd.ny, gd.nz, xmin, ymin,
+                     zmin, dx, dy, dz, natom, xyz_d, type_d, scale_d, npot_d, vr_d, pot_d, stride,
+                     dr_d, ex_e, ex_x0, ex_t, potl_d);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
+// ------------------------------------------------ deterministic row dots
+// stage 1: part[i*nb + b] = sum over block b's chunk; stage 2: fixed-order sum
+template <typename R>
+__global__ void __launch_bounds__(256) k_dot_part(const R *a, long lda, const R *b, long ldb,
+                                                  long n, long chunk, double *part) {
+  const long i = blockIdx.y;
+  const long b0 = (long)blockIdx.x * chunk;
+  long b1 = b0 + chunk;
+  if (b1 > n) b1 = n;
+  double s = 0.0;
+  for (long g = b0 + threadIdx.x; g < b1; g += 256) s += (double)a[i * lda + g] * (double)b[i * ldb + g];
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[i * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__global__ void k_dot_final(const double *part, long nb, long m, double *out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (long b = 0; b < nb; b++) s += part[i * nb + b];
+  out[i] = s;
+}
+
+template <typename R>
+static int dots_impl(const R *a, long lda, const R *b, long ldb, long m, long n, double *out_d,
+                     double *scratch, long scratch_len, hipStream_t st) {
+  long chunk = 16384;
+  long nb = (n + chunk - 1) / chunk;
+  if (nb * m > scratch_len) return SEP_ERR_NOMEM;
+  hipLaunchKernelGGL(k_dot_part<R>, dim3((unsigned)nb, (unsigned)m), dim3(256), 0, st, a, lda, b,
+                     ldb, n, chunk, scratch);
+  hipLaunchKernelGGL(k_dot_final, dim3((unsigned)((m + 127) / 128)), dim3(128), 0, st, scratch, nb,
+                     m, out_d);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+int dots_dev(const double *a, long lda, const double *b, long ldb, long m, long n, double *out_d,
+             double *scratch, long scratch_len, hipStream_t st) {
+  return dots_impl<double>(a, lda, b, ldb, m, n, out_d, scratch, scratch_len, st);
+}
+int dots_dev_f(const float *a, long lda, const float *b, long ldb, long m, long n, double *out_d,
+               double *scratch, long scratch_len, hipStream_t st) {
+  return dots_impl<float>(a, lda, b, ldb, m, n, out_d, scratch, scratch_len, st);
+}
+
+__global__ void k_scale_rows(double *x, long ld, long n, const double *s) {
+  const long i = blockIdx.y;
+  const double f = s[i];
+  for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    x[i * ld + g] *= f;
+}
+int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStream_t st) {
+  unsigned nbx = (unsigned)((n + 255) / 256);
+  if (nbx > 4096) nbx = 4096;
+  hipLaunchKernelGGL(k_scale_rows, dim3(nbx, (unsigned)m), dim3(256), 0, st, x, ld, n, s_d);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
+__global__ void k_d2f(const double *in, float *out, long n) {
+  for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    out[g] = (float)in[g];
+}
+__global__ void k_f2d(const float *in, double *out, long n) {
+  for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    out[g] = (double)in[g];
+}
+int convert_d2f(const double *in, float *out, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(256), 0, st, in, out, n);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+int convert_f2d(const float *in, double *out, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_f2d, dim3(4096), dim3(256), 0, st, in, out, n);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
+// --------------------------------------------- K-blocked Newton accumulation
+// filter/filter.c:74-78 applied to `cnt` stored Newton vectors at once:
+//   acc[s][ie][g] += sum_k an[ie][j0+k] * ring[k][s][g]
+// The k sum is done in the reference's order (ascending j) directly into the
+// accumulator.  Targets are processed in register chunks of MSB.
+template <typename R, int MSB>
+__global__ void __launch_bounds__(256) k_accum(const R *__restrict__ ring, long slot, int cnt,
+                                               const double *__restrict__ an, long nc, long j0,
+                                               int ms, double *__restrict__ acc, long N) {
+  extern __shared__ double a_s[];  // [ms][cnt]
+  for (int t = threadIdx.x; t < ms * cnt; t += blockDim.x) {
+    int ie = t / cnt, k = t % cnt;
+    a_s[t] = an[(long)ie * nc + j0 + k];
+  }
+  __syncthreads();
+  const long s = blockIdx.y;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  const R *rp = ring + s * N + g;
+  for (int i0 = 0; i0 < ms; i0 += MSB) {
+    double a[MSB];
+    double *ap = acc + (s * ms + i0) * N + g;
+#pragma unroll
+    for (int q = 0; q < MSB; q++) a[q] = (i0 + q < ms) ? ap[(long)q * N] : 0.0;
+    for (int k = 0; k < cnt; k++) {
+      const double x = (double)rp[(long)k * slot];
+#pragma unroll
+      for (int q = 0; q < MSB; q++) a[q] += a_s[(i0 + q < ms ? i0 + q : 0) * cnt + k] * x;
+    }
+#pragma unroll
+    for (int q = 0; q < MSB; q++)
+      if (i0 + q < ms) ap[(long)q * N] = a[q];
+  }
+}
+
+template <typename R>
+int accum_dev(const R *ring, long slot, int cnt, const double *an_d, long nc, long j0, int ms,
+              double *acc, int ns, long N, hipStream_t st) {
+  dim3 grid((unsigned)((N + 255) / 256), (unsigned)ns);
+  size_t lds = sizeof(double) * (size_t)ms * cnt;
+  if (lds > 64 * 1024) return SEP_ERR_ARG;
+  hipLaunchKernelGGL((k_accum<R, 16>), grid, dim3(256), lds, st, ring, slot, cnt, an_d, nc, j0, ms,
+                     acc, N);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+template int accum_dev<double>(const double *, long, int, const double *, long, long, int,
+                               double *, int, long, hipStream_t);
+template int accum_dev<float>(const float *, long, int, const double *, long, long, int, double *,
+                              int, long, hipStream_t);
+
+}  // namespace sep

@@ -1,0 +1,282 @@
+/*
+ * filter_main.c -- the `filter` program on libsep: a drop-in for the
+ * reference's filter.x (filter/main.c:11-162).  Reads input.par, conf.par,
+ * pot*.par (+ optional expot.par, strain.par, seed.par) from the working
+ * directory (or --dir), writes eval.dat, psi.dat and the reference's side
+ * files eval-filt-0-<jns>.dat, Egrid.dat, func.dat.
+ *
+ * Options (the reference ignores argv; these are extensions):
+ *   --dir D            deck / output directory (default .)
+ *   --seed S           ran_nrc seed (overrides seed.par; 0 = clock as main.c:79)
+ *   --precision P      fp64 (default) or fp32 (complex64 filter)
+ *   --device N         HIP device ordinal
+ *   --psi-filt         also write psi-filt-0-<jns>.dat (filter/filter.c:44-48)
+ * Failures write the message to a file named `error` and exit 1, as the
+ * reference's nerror() does (filter/nerror.c:9-16).
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <sys/time.h>
+
+#include "../../include/sep.h"
+#include "../../include/sep_deck.h"
+
+static const char *g_dir = ".";
+
+static FILE *out_file(const char *name, const char *mode) {
+  char path[4096];
+  snprintf(path, sizeof path, "%s/%s", g_dir, name);
+  return fopen(path, mode);
+}
+
+static void die(const char *msg) {
+  FILE *pf = out_file("error", "w");
+  if (pf) {
+    fprintf(pf, "%s\n", msg);
+    fclose(pf);
+  }
+  fprintf(stderr, "filter: %s\n", msg);
+  exit(1);
+}
+
+static double wall(void) {
+  struct timeval tv;
+  gettimeofday(&tv, NULL);
+  return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+static void stamp(void) {
+  time_t t = time(NULL);
+  fputs(ctime(&t), stdout);
+}
+static void separation(void) {
+  printf("\n******************************************************************************\n\n");
+}
+
+/* check_function, filter/coeff.c:138-163: Newton interpolant vs the exact
+ * Gaussian on [-2, 2] for the first target. */
+static void write_func(const double *an0, const double *zn, long nc, double dE, double vmin,
+                       double dt, double el0) {
+  FILE *pf = out_file("func.dat", "w");
+  if (!pf) return;
+  for (double x = -2.0; x < 2.0; x += 0.01) {
+    double xn = 1.0, f = an0[0];
+    for (long j = 1; j < nc; j++) {
+      xn *= (x - zn[j - 1]);
+      f += an0[j] * xn;
+    }
+    double xu = (x + 2.0) * dE / 4.0 + vmin;
+    double g = sqrt(dt / 3.14159265358979323846) * exp(-((xu - el0) * (xu - el0)) * dt);
+    fprintf(pf, "%g %g %g\n", xu, f, g);
+  }
+  fclose(pf);
+}
+
+int main(int argc, char **argv) {
+  long seed_opt = 0;
+  int have_seed_opt = 0, prec = SEP_PREC_FP64, device = 0, psi_filt = 0;
+  for (int i = 1; i < argc; i++) {
+    if (!strcmp(argv[i], "--dir") && i + 1 < argc)
+      g_dir = argv[++i];
+    else if (!strcmp(argv[i], "--seed") && i + 1 < argc) {
+      seed_opt = atol(argv[++i]);
+      have_seed_opt = 1;
+    } else if (!strcmp(argv[i], "--precision") && i + 1 < argc) {
+      const char *p = argv[++i];
+      if (!strcmp(p, "fp64"))
+        prec = SEP_PREC_FP64;
+      else if (!strcmp(p, "fp32"))
+        prec = SEP_PREC_FP32;
+      else
+        die("--precision must be fp64 or fp32");
+    } else if (!strcmp(argv[i], "--device") && i + 1 < argc)
+      device = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--psi-filt"))
+      psi_filt = 1;
+  }
+  stamp();
+  separation();
+  char err[512];
+  sep_deck d;
+  if (sep_deck_read(g_dir, &d, err, sizeof err) != SEP_OK) die(err);
+  const long N = d.nx * d.ny * d.nz, ms = d.ms, ns = d.ns, nc = d.nc;
+  /* filter/init.c:40-47 */
+  printf("Number of atoms in the system, natom = %ld\n", d.natom);
+  printf("Number of grid points used: nx = %ld  ny = %ld  nz = %ld\n", d.nx, d.ny, d.nz);
+  printf("Number of states per filter, ms = %ld\nNumber of filter cycles, ns = %ld\n", ms, ns);
+  printf("Length of Newton interpolation used, nc = %ld\n", nc);
+  printf("Target energy for HOMO state = % .4f\n", d.Elmin);
+  printf("Target energy for LUMO state = % .4f\n", d.Elmax);
+  printf("Kinetic energy maximum = % .2f\n", 10.0);
+  printf("Number of openMP threads used = %ld\n", d.nthreads);
+  printf("Precision: %s\n", prec == SEP_PREC_FP64 ? "fp64" : "fp32 filter + fp64 subspace");
+  if (d.has_expot)
+    printf("expot.par was detected: e = % .6f x0 = % .6f kT = % .6f\n", d.expot[0], d.expot[1],
+           d.expot[2]);
+  else
+    printf("No expot.par file detected in cwd - setting external potential to 0!\n");
+  if (d.has_strain) printf("Reading strain scale from strain.par...\n");
+  printf("Box (quadrant) dimensions: xd = %.2f yd = %.2f zd = %.2f\n", d.xd, d.yd, d.zd);
+  printf("Grid point spacing: dx = %.4f dy = %.4f dz = %.4f dv = %.6f dr = %.6f\n", d.dx, d.dy,
+         d.dz, d.dv, sqrt(d.dx * d.dx + d.dy * d.dy + d.dz * d.dz));
+  fflush(stdout);
+  {
+    const long dims[3] = {d.nx, d.ny, d.nz};
+    for (int a = 0; a < 3; a++)
+      if (!sep_fft_length_supported(dims[a])) {
+        snprintf(err, sizeof err, "grid length %ld has no HIP FFT plan", dims[a]);
+        die(err);
+      }
+  }
+
+  sep_grid_desc gdesc = {d.nx, d.ny, d.nz, d.dx, d.dy, d.dz, d.xmin, d.ymin, d.zmin, 10.0, device};
+  sep_ctx *ctx = NULL;
+  int rc = sep_open(&gdesc, &ctx);
+  if (rc != SEP_OK) {
+    snprintf(err, sizeof err, "sep_open failed (%d)", rc);
+    die(err);
+  }
+#define CHECK(call)                                                          \
+  do {                                                                       \
+    int rc_ = (call);                                                        \
+    if (rc_ != SEP_OK) {                                                     \
+      snprintf(err, sizeof err, "%s: %s (%d)", #call, sep_last_error(ctx), rc_); \
+      die(err);                                                              \
+    }                                                                        \
+  } while (0)
+
+  /* V(r) on the device (filter/init.c:157-199) */
+  int *ti = (int *)malloc(sizeof(int) * d.natom);
+  for (long i = 0; i < d.natom; i++) ti[i] = d.type[i];
+  double Vmin, Vmax;
+  CHECK(sep_build_potential(ctx, d.natom, d.xyz, ti, d.has_strain ? d.scale : NULL, SEP_NTYPE,
+                            d.npot, d.vr, d.pot, SEP_NPOT, d.dr, d.expot, NULL, &Vmin, &Vmax));
+  free(ti);
+  printf("Potential energy statistics: dV = %.5f Vmin = % .5f Vmax = %.5f\n", Vmax - Vmin, Vmin,
+         Vmax);
+  const double PIE = 3.14159265358979323846;
+  printf("dT = %g\n", 0.5 * PIE * PIE / (d.dx * d.dx) + 0.5 * PIE * PIE / (d.dy * d.dy) +
+                          0.5 * PIE * PIE / (d.dz * d.dz));
+  /* energy targets, filter/init.c:209-214 */
+  double *el = (double *)malloc(sizeof(double) * ms);
+  {
+    double del = ms > 1 ? (d.Elmax - d.Elmin) / (double)(ms - 1) : 0.0;
+    FILE *pf = out_file("Egrid.dat", "w");
+    for (long j = 0; j < ms; j++) {
+      el[j] = d.Elmin + (double)j * del;
+      if (pf) fprintf(pf, "%g\n", el[j]);
+    }
+    if (pf) fclose(pf);
+  }
+  /* spectral range (filter/main.c:63-66) */
+  double t0 = wall(), c0 = (double)clock();
+  double dE, Emax;
+  long iters;
+  CHECK(sep_energy_range(ctx, Vmin, &dE, &Emax, &iters));
+  printf("dE = %g\n", dE);
+  printf("done calculate energy range, CPU time (sec) %g, wall run time (sec) %g\n",
+         ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
+  /* Newton interpolation (main.c:70-74) */
+  const double dt = ((double)nc / (2.5 * dE)) * ((double)nc / (2.5 * dE));
+  printf("nc = %ld dt = %g dE = %g\n", nc, dt, dE);
+  fflush(stdout);
+  double *zn = (double *)malloc(sizeof(double) * nc);
+  double *an = (double *)malloc(sizeof(double) * nc * ms);
+  sep_leja_points(nc, zn);
+  if (sep_newton_coeffs(nc, ms, zn, el, dE, Vmin, dt, an) != SEP_OK) die("coefficient failed");
+  FILE *pc = out_file("coeff.dat", "w");
+  if (pc) fclose(pc);
+  write_func(an, zn, nc, dE, Vmin, dt, el[0]);
+  /* random states (main.c:79-89) */
+  long idum;
+  long s = have_seed_opt ? seed_opt : (d.has_seed ? d.seed : 0);
+  if (s == 0) {
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    srandom((unsigned)(tv.tv_sec ^ tv.tv_usec));
+    idum = -random();
+  } else {
+    idum = s < 0 ? s : -s;
+  }
+  printf("seed = %ld\n", idum);
+  fflush(stdout);
+  double *psi0 = (double *)malloc(sizeof(double) * ns * N);
+  sep_rng rng;
+  sep_rng_reset(&rng);
+  for (long j = 0; j < ns; j++) sep_init_psi(&rng, &idum, N, d.dv, psi0 + j * N);
+  /* filter (main.c:91-100) */
+  const long mstot = ms * ns;
+  double *ene = (double *)malloc(sizeof(double) * mstot);
+  double *psif = psi_filt ? (double *)malloc(sizeof(double) * mstot * N) : NULL;
+  t0 = wall();
+  c0 = (double)clock();
+  CHECK(sep_filter(ctx, psi0, ns, an, zn, nc, ms, Vmin, dE, prec, psif, ene));
+  printf("done calculating filter, CPU time (sec) %g, wall run time (sec) %g\n",
+         ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
+  for (long jns = 0; jns < ns; jns++) { /* filter.c:38-48 */
+    char name[64];
+    snprintf(name, sizeof name, "eval-filt-0-%ld.dat", jns);
+    FILE *pf = out_file(name, "w");
+    if (pf) {
+      for (long j = 0; j < ms; j++) fprintf(pf, "%ld %.16g %.16g\n", j, ene[jns * ms + j], el[j]);
+      fclose(pf);
+    }
+    if (psif) {
+      snprintf(name, sizeof name, "psi-filt-0-%ld.dat", jns);
+      pf = out_file(name, "wb");
+      if (pf) {
+        fwrite(psif + jns * ms * N, sizeof(double), ms * N, pf);
+        fclose(pf);
+      }
+    }
+  }
+  free(psif);
+  /* ortho (main.c:109-114) */
+  t0 = wall();
+  c0 = (double)clock();
+  long kept = 0;
+  double *S = (double *)malloc(sizeof(double) * mstot);
+  CHECK(sep_ortho(ctx, mstot, 1.0e-10, S, &kept));
+  printf("cutoff is %ld\n", kept);
+  printf("mstot ortho = %ld\n", kept);
+  printf("done calculating ortho, CPU time (sec) %g, wall run time (sec) %g\n",
+         ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
+  /* Rayleigh-Ritz (main.c:120-126) */
+  t0 = wall();
+  c0 = (double)clock();
+  double *eval = (double *)malloc(sizeof(double) * kept);
+  double *sige = (double *)malloc(sizeof(double) * kept);
+  CHECK(sep_rayleigh_ritz(ctx, kept, eval));
+  printf("done calculating Hmat, CPU time (sec) %g, wall run time (sec) %g\n",
+         ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
+  fflush(stdout);
+  /* psi.dat (main.c:129-131) */
+  {
+    double *psi = (double *)malloc(sizeof(double) * kept * N);
+    if (!psi) die("psi.dat buffer");
+    CHECK(sep_states_download(ctx, psi, kept));
+    FILE *pf = out_file("psi.dat", "wb");
+    if (!pf) die("cannot write psi.dat");
+    fwrite(psi, sizeof(double), kept * N, pf);
+    fclose(pf);
+    free(psi);
+  }
+  /* sigma and eval.dat (main.c:135-142) */
+  CHECK(sep_sigma(ctx, kept, sige));
+  {
+    FILE *pf = out_file("eval.dat", "w");
+    if (!pf) die("cannot write eval.dat");
+    for (long j = 0; j < kept; j++) fprintf(pf, "%ld %.16g %g\n", j, eval[j], sige[j]);
+    fclose(pf);
+  }
+  sep_close(ctx);
+  sep_deck_free(&d);
+  free(el); free(zn); free(an); free(psi0); free(ene); free(S); free(eval); free(sige);
+  separation();
+  stamp();
+  return 0;
+}

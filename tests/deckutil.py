@@ -1,0 +1,36 @@
+"""Test helpers: deck -> potential on both the oracle and libsep."""
+import numpy as np
+
+
+def oracle_potential(oracle, d):
+    """V(r) of a parsed deck (semi_empirical_pseudopotentials_amd._lib.read_deck) by the oracle."""
+    grid = (d["nx"], d["ny"], d["nz"])
+    return oracle.build_potential(grid, (d["xmin"], d["ymin"], d["zmin"]),
+                                  (d["dx"], d["dy"], d["dz"]), d["xyz"], d["type"], d["scale"],
+                                  d["npot"], d["vr"], d["pot"], d["dr"], tuple(d["expot"]))
+
+
+def context_for(sep, d):
+    return sep.Context(d["nx"], d["ny"], d["nz"], d["dx"], d["dy"], d["dz"], d["xmin"],
+                       d["ymin"], d["zmin"])
+
+
+def device_potential(ctx, d):
+    return ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
+                               scale=d["scale"] if d["has_strain"] else None,
+                               expot=d["expot"])
+
+
+def read_eval(path):
+    a = np.loadtxt(path, ndmin=2)
+    return a[:, 1], a[:, 2]
+
+
+def match_levels(ref_e, ref_s, got_e, sig_max):
+    """For every reference level with sigma < sig_max, the nearest got level."""
+    out = []
+    for e, s in zip(ref_e, ref_s):
+        if s < sig_max:
+            j = np.argmin(np.abs(got_e - e))
+            out.append((e, got_e[j]))
+    return np.array(out)
