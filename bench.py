@@ -1,0 +1,202 @@
+#!/usr/bin/env python
+"""H.psi applications/s of the filter-diagonalization hot loop on one MI355X.
+
+Workload (the production configuration the metric is quoted on): a 4.0 nm
+zincblende CdSe dot (1667 atoms incl. pseudo-ligands), 128^3 grid, 16 seeded
+random states x 32 energy targets, Newton length 4096.  One step = one full
+filter pass (filter/filter.c:59-83 for all 16 states: 16 x 4095 H.psi with
+the Newton recurrence and the 32-target accumulation, then normalize_all)
+with the random states resident in HBM.  The headline is the complex64
+filter; the FP64 filter is timed the same way beside it.
+
+value = states x (nc - 1) x steps / time, i.e. reference-equivalent H.psi
+applications (each real state's application counts once; the kernels carry
+two real states per complex vector because H is real).
+
+Prints one JSON line (rank 0, N = 1).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("H·ψ applications/s at 128³; FP32+refine vs FP64 GPU at "
+          "≤1 meV error")
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--grid", type=int, default=128)
+    p.add_argument("--diameter", type=float, default=4.0)
+    p.add_argument("--ns", type=int, default=16)
+    p.add_argument("--ms", type=int, default=32)
+    p.add_argument("--nc", type=int, default=4096)
+    p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-steps", type=int, default=24)
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the filter "
+              "would shard random states across GPUs, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import semi_empirical_pseudopotentials_amd as sep
+    from semi_empirical_pseudopotentials_amd import decks
+    from semi_empirical_pseudopotentials_amd._lib import read_deck
+
+    n, ns, ms, nc = args.grid, args.ns, args.ms, args.nc
+    # ---------------------------------------------------------------- setup
+    t_setup = time.time()
+    syms, xyz = decks.cdse_dot(args.diameter)
+    tmp = tempfile.mkdtemp(prefix="sep_bench_")
+    elmid = -0.215  # mid-gap of this dot (FP64 exploratory run), +-2 eV window
+    decks.write_deck(tmp, syms, xyz, n, n, n, ms, ns, nc, elmid - 0.0735, elmid + 0.0735)
+    d = read_deck(tmp)
+    ctx = sep.Context(n, n, n, d["dx"], d["dy"], d["dz"], d["xmin"], d["ymin"], d["zmin"])
+    ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
+                        expot=d["expot"])
+    vmin = ctx.vmin
+    dE, emax, iters = ctx.energy_range()
+    zn, _ = sep.leja_points(nc)
+    el = np.linspace(d["Elmin"], d["Elmax"], ms)
+    dt = (nc / (2.5 * dE)) ** 2
+    an = sep.newton_coeffs(zn, el, dE, vmin, dt)
+    N = n ** 3
+    psi0 = np.empty((ns, N))
+    idum, rng = -1002, None
+    for j in range(ns):
+        psi0[j], idum, rng = sep.init_psi(idum, N, d["dv"], rng)
+    psi0_dev = torch.from_numpy(psi0).to("cuda:0")
+    del psi0
+    torch.cuda.synchronize()
+    log(f"setup {time.time() - t_setup:.1f}s: natom={d['natom']} grid={n}^3 dx={d['dx']:.4f} "
+        f"dE={dE:.4f} (power iterations {iters})")
+
+    # ------------------------------------------------ per-pass kernel timing
+    nvec = (ns + 1) // 2
+    kern = {}
+    for prec, name in ((sep.SEP_PREC_FP32, "fp32"), (sep.SEP_PREC_FP64, "fp64")):
+        if name == "fp64" and args.no_fp64:
+            continue
+        kern[name] = ctx.bench_hpsi(prec, nvec, ms, 20)
+        log(f"{name} pass ms: A {kern[name][0]:.4f} B {kern[name][1]:.4f} "
+            f"C {kern[name][2]:.4f} accum(K=64) {kern[name][3]:.4f} A+B+C {kern[name][4]:.4f}")
+
+    # ------------------------------------------------------------ timed loop
+    def run(prec):
+        for _ in range(args.warmup):
+            ctx.filter_dev(psi0_dev.data_ptr(), ns, an, zn, vmin, dE, prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.filter_dev(psi0_dev.data_ptr(), ns, an, zn, vmin, dE, prec)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    t32 = run(sep.SEP_PREC_FP32)
+    hpsi = ns * (nc - 1) * args.steps
+    v32 = hpsi / t32
+    log(f"fp32 filter: {t32 / args.steps:.3f} s/step, {v32:.0f} H.psi/s")
+    v64 = None
+    if not args.no_fp64:
+        t64 = run(sep.SEP_PREC_FP64)
+        v64 = hpsi / t64
+        log(f"fp64 filter: {t64 / args.steps:.3f} s/step, {v64:.0f} H.psi/s")
+
+    # ------------------------------------------------------------- roofline
+    # algorithmic bytes per launch (nvec complex vectors of N points):
+    #   pass A: read psi (re, im planes) + write T          2c N nvec
+    #   pass B: read T + write T                             2c N nvec
+    #   pass C: read T, psi, V + write psi_j                 (3c + r) N nvec
+    # c = complex bytes, r = real bytes (V).
+    def roof(name, c, r):
+        k = kern[name]
+        per = {"A": 2 * c * N * nvec, "B": 2 * c * N * nvec, "C": (3 * c + r) * N * nvec}
+        out = {}
+        for i, key in enumerate("ABC"):
+            gbs = per[key] / (k[i] * 1e-3) / 1e9
+            out[key] = {"ms": k[i], "bytes": per[key], "GBs": gbs, "frac": gbs / PEAK_HBM_GBS}
+        tot = sum(per.values())
+        out["ABC"] = {"ms": k[4], "bytes": tot, "GBs": tot / (k[4] * 1e-3) / 1e9,
+                      "frac": tot / (k[4] * 1e-3) / 1e9 / PEAK_HBM_GBS}
+        return out
+
+    r32 = roof("fp32", 8, 4)
+    dom = max("ABC", key=lambda q: r32[q]["ms"])
+    kname = {"A": "k_plane_fwd", "B": "k_pencil", "C": "k_plane_bwd"}[dom]
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(r32[dom]["GBs"], 1),
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(r32[dom]["frac"], 4),
+                "traffic": None,
+                "hpsi_passes_frac": round(r32["ABC"]["frac"], 4)}
+
+    # ---------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        grid = (n, n, n)
+        ksq = O.ksqr(*grid, d["dx"], d["dy"], d["dz"])
+        V = np.empty(N)
+        V[:] = ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
+                                   expot=d["expot"])
+        steps = args.cpu_steps
+        p0 = psi0_dev[0].cpu().numpy()
+        t0 = time.perf_counter()
+        O.filter_state(grid, ksq, V, an[:, :steps + 1], zn[:steps + 1], 4.0 / dE, vmin, p0)
+        tc = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": round(steps / tc, 3), "unit": "H·ψ/s", "cores": threads,
+               "kind": "port",
+               "sample": f"oracle filter (own C FFT, OpenMP), 1 state x {steps} Newton steps "
+                         f"x {ms} targets at {n}^3, {tc:.1f} s"}
+        log(f"cpu oracle: {steps / tc:.2f} H.psi/s on {threads} threads")
+
+    res = {
+        "metric": METRIC,
+        "value": round(v32, 1),
+        "unit": "H·ψ/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t32 / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"CdSe {args.diameter} nm dot ({d['natom']} atoms), {n}^3 grid, "
+                               f"{ns} random states x {ms} targets, Newton length {nc}; "
+                               "complex64 filter (FP64 accumulators)",
+                   "grid": n, "states": ns, "targets": ms, "newton_length": nc},
+        "fp64_value": round(v64, 1) if v64 else None,
+        "fp32_over_fp64": round(v32 / v64, 3) if v64 else None,
+        "complex_hpsi_per_s": round(v32 / 2, 1),
+        "roofline": roofline,
+        "roofline_passes_fp32": r32,
+        "roofline_passes_fp64": roof("fp64", 16, 8) if "fp64" in kern else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

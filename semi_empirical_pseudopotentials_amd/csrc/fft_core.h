@@ -22,6 +22,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace sep {
 
 template <typename R>
@@ -283,6 +285,101 @@ __device__ __forceinline__ void compute_pass(cx<R> *v, int g, const cx<R> *tw) {
   }
 }
 
+// LDS position of line element i: one pad slot every 8 elements, so the
+// stride-8 access patterns of the radix-8/4 exchanges spread over the banks.
+__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 3); }
+// The same spreading without extra space: the low three bits are xor-ed
+// with the next three (a permutation inside each aligned block of 8).
+__host__ __device__ constexpr int lds_swz(int i) { return i ^ ((i >> 3) & 7); }
+template <int N>
+constexpr int lds_line() {
+  return lds_pad(N - 1) + 1;
+}
+
+// Per-thread twiddle cache: the factors w_N^{k t N/(Ns r)} a thread needs in
+// passes 1..P-1 depend only on its index g, so they are read from the table
+// once per kernel and then reused for every line the thread transforms.
+template <int N, int S>
+struct TwLayout {
+  using PL = FftPlan<N>;
+  static constexpr int off(int p) {
+    int o = 0;
+    for (int q = 1; q < p; q++) o += (PL::E / Pass<N, S>::radix(q)) * (Pass<N, S>::radix(q) - 1);
+    return o;
+  }
+  static constexpr int total = off(PL::P) > 0 ? off(PL::P) : 1;
+};
+
+template <int N, int S, typename R>
+struct TwCache {
+  cx<R> w[TwLayout<N, S>::total];
+  template <int p>
+  __device__ __forceinline__ void fill(int g, const cx<R> *tw) {
+    if constexpr (p < FftPlan<N>::P) {
+      using PL = FftPlan<N>;
+      constexpr int r = Pass<N, S>::radix(p);
+      constexpr int Ns = Pass<N, S>::ns(p);
+      constexpr int o = TwLayout<N, S>::off(p);
+#pragma unroll
+      for (int b = 0; b < PL::E / r; b++) {
+        const int t = (g + PL::G * b) % Ns;
+#pragma unroll
+        for (int k = 1; k < r; k++) {
+          cx<R> x = tw[k * t * (N / (Ns * r))];
+          if constexpr (S > 0) x.im = -x.im;
+          w[o + b * (r - 1) + (k - 1)] = x;
+        }
+      }
+      fill<p + 1>(g, tw);
+    }
+  }
+  __device__ __forceinline__ void init(int g, const cx<R> *tw) { fill<1>(g, tw); }
+  template <int p>
+  __device__ __forceinline__ cx<R> get(int b, int k) const {
+    constexpr int r = Pass<N, S>::radix(p);
+    return w[TwLayout<N, S>::off(p) + b * (r - 1) + (k - 1)];
+  }
+};
+
+// The same interface, reading each twiddle from the (LDS) table when it is
+// used: fewer registers, for the FP64 kernels whose cache would spill.
+template <int N, int S, typename R>
+struct TwTable {
+  const cx<R> *tw;
+  int g;
+  __device__ __forceinline__ void init(int g_, const cx<R> *tw_) { g = g_; tw = tw_; }
+  template <int p>
+  __device__ __forceinline__ cx<R> get(int b, int k) const {
+    constexpr int r = Pass<N, S>::radix(p);
+    constexpr int Ns = Pass<N, S>::ns(p);
+    const int t = (g + FftPlan<N>::G * b) % Ns;
+    cx<R> x = tw[k * t * (N / (Ns * r))];
+    if constexpr (S > 0) x.im = -x.im;
+    return x;
+  }
+};
+
+// Twiddle source of the plane and row kernels: registers for FP32, the
+// table for FP64 (a cache of 64-bit twiddles costs up to 96 VGPRs).
+template <int N, int S, typename R>
+using TwFor = typename std::conditional<sizeof(R) == 4, TwCache<N, S, R>, TwTable<N, S, R>>::type;
+
+// compute_pass with the twiddles taken from a twiddle source (TwCache / TwTable).
+template <int N, int S, int p, typename R, typename Tw>
+__device__ __forceinline__ void compute_pass_c(cx<R> *v, const Tw &tc) {
+  using PL = FftPlan<N>;
+  constexpr int r = Pass<N, S>::radix(p);
+  constexpr int nb = PL::E / r;
+#pragma unroll
+  for (int b = 0; b < nb; b++) {
+    if constexpr (p > 0) {
+#pragma unroll
+      for (int k = 1; k < r; k++) v[b * r + k] = cmul(v[b * r + k], tc.template get<p>(b, k));
+    }
+    dft<r, S>(v + b * r);
+  }
+}
+
 // Exchange through a line buffer: write pass-p outputs, barrier, read pass
 // p+1 inputs.  `at(i)` returns a reference into the line buffer.  The caller
 // guarantees a barrier before the write if the buffer was being read.
@@ -309,20 +406,51 @@ __device__ __forceinline__ void read_in(cx<R> *v, int g, bool act, At at) {
   }
 }
 
-// Runs passes p0..p1-1 (p0 >= 0) with LDS exchanges between them.  On entry
-// v holds the pass-p0 inputs; on exit v holds the outputs of pass p1-1.
-// Every thread of the block must call this (it contains __syncthreads).
-template <int N, int S, int p, int pend, typename R, typename At>
+// Synchronisation between the passes of one line: a block barrier, or a
+// wavefront barrier when every line lives inside one wavefront (LDS
+// operations of a wavefront complete in order; the fences and the scheduling
+// barrier keep the compiler from moving accesses across it).
+struct BlockSync {
+  static __device__ __forceinline__ void sync() { __syncthreads(); }
+};
+struct WaveSync {
+  static __device__ __forceinline__ void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
+// Runs passes p..pend-1 with LDS exchanges between them.  On entry v holds
+// the pass-p inputs; on exit v holds the outputs of pass pend-1.  With
+// BlockSync every thread of the block must call this.
+template <int N, int S, int p, int pend, typename Sync = BlockSync, typename R, typename At>
 __device__ __forceinline__ void run_passes(cx<R> *v, int g, bool act, const cx<R> *tw, At at) {
   if constexpr (p < pend) {
     compute_pass<N, S, p>(v, g, tw);
     if constexpr (p + 1 < pend) {
-      __syncthreads();  // readers of the buffer are done
+      Sync::sync();  // readers of the buffer are done
       write_out<N, S, p>(v, g, act, at);
-      __syncthreads();
+      Sync::sync();
       read_in<N, S, p + 1>(v, g, act, at);
     }
-    run_passes<N, S, p + 1, pend>(v, g, act, tw, at);
+    run_passes<N, S, p + 1, pend, Sync>(v, g, act, tw, at);
+  }
+}
+
+// run_passes with the twiddles of a twiddle source (TwCache / TwTable).
+template <int N, int S, int p, int pend, typename Sync = BlockSync, typename R, typename Tw,
+          typename At>
+__device__ __forceinline__ void run_passes_c(cx<R> *v, int g, bool act, const Tw &tc, At at) {
+  if constexpr (p < pend) {
+    compute_pass_c<N, S, p>(v, tc);
+    if constexpr (p + 1 < pend) {
+      Sync::sync();
+      write_out<N, S, p>(v, g, act, at);
+      Sync::sync();
+      read_in<N, S, p + 1>(v, g, act, at);
+    }
+    run_passes_c<N, S, p + 1, pend, Sync>(v, g, act, tc, at);
   }
 }
 

@@ -535,10 +535,10 @@ static int filter_impl(sep_ctx *c, const double *psi0, bool psi0_on_device, long
   const double dE_1 = 4.0 / dE;
   const long nvec = (ns + 1) / 2, S2 = 2 * nvec;
   // ring of K Newton vectors: K * S2 * N reals
-  long K = nc < 64 ? nc : 64;
+  // K Newton vectors are accumulated per pass over the ring; the
+  // accumulators move 2 ms 8 / K bytes per real-state point per step
+  long K = nc < 256 ? nc : 256;
   while (K > 8 && (double)K * S2 * N * sizeof(R) > 64.0e9) K /= 2;
-  if ((long)ms * K * 8 > 64 * 1024) K = (64 * 1024) / (8 * ms);
-  if (K < 1) return fail(c, SEP_ERR_ARG, "ms too large for the accumulation kernel");
   const long slot = S2 * N;
   RCHK(ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * slot));
   R *ring = reinterpret_cast<R *>(c->ring);
@@ -556,9 +556,13 @@ static int filter_impl(sep_ctx *c, const double *psi0, bool psi0_on_device, long
     HIPCHK(c, hipMemcpyAsync(c->states2, psi0, sizeof(double) * ns * N, kind, c->st));
     RCHK(convert_d2f(c->states2, (float *)ring, ns * N, c->st));
   }
+  // coefficients transposed to [j][ie] for the accumulation kernel
+  std::vector<double> anT((size_t)ms * nc);
+  for (long ie = 0; ie < ms; ie++)
+    for (long j = 0; j < nc; j++) anT[j * ms + ie] = an[ie * nc + j];
   double *an_d = nullptr;
   HIPCHK(c, hipMalloc(&an_d, sizeof(double) * ms * nc));
-  HIPCHK(c, hipMemcpyAsync(an_d, an, sizeof(double) * ms * nc, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(an_d, anT.data(), sizeof(double) * ms * nc, hipMemcpyHostToDevice, c->st));
   int rc = SEP_OK;
   for (long j = 1; j < nc && rc == SEP_OK; j++) {
     const long sp = (j - 1) % K, sj = j % K;
@@ -568,11 +572,11 @@ static int filter_impl(sep_ctx *c, const double *psi0, bool psi0_on_device, long
     const double beta = 2.0 + zn[j - 1] + vmin * dE_1;
     rc = apply_views<R>(c, in, out, (int)nvec, dE_1, beta, nullptr);
     if (rc == SEP_OK && (sj == K - 1 || j == nc - 1))
-      rc = accum_dev<R>(ring, slot, (int)(sj + 1), an_d, nc, j - sj, (int)ms, c->states, (int)ns,
-                        N, c->st);
+      rc = accum_mfma_dev<R>(ring, slot, (int)(sj + 1), an_d, j - sj, (int)ms, c->states, (int)ns, N,
+                        c->st);
   }
-  if (rc == SEP_OK && nc == 1) rc = accum_dev<R>(ring, slot, 1, an_d, nc, 0, (int)ms, c->states, (int)ns, N, c->st);
-  hipStreamSynchronize(c->st);
+  if (rc == SEP_OK && nc == 1) rc = accum_mfma_dev<R>(ring, slot, 1, an_d, 0, (int)ms, c->states, (int)ns, N, c->st);
+  hipStreamSynchronize(c->st);  // anT must outlive the async copy
   hipFree(an_d);
   if (rc != SEP_OK) return rc;
   return normalize_dev(c, c->states, ns * ms);  // filter.c:33
@@ -755,7 +759,7 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
   for (long it = -1; it < iters && rc == SEP_OK; it++) {  // it = -1: warm-up
     rc = hpsi_apply_timed<R>(c->gd, in, out, tb, pot_of<R>(c), (int)nvec, 0.3, 1.7, c->st, ev);
     if (rc == SEP_OK)
-      rc = accum_dev<R>(ring, S2 * N, (int)K, an_d, K, 0, (int)ms, c->states, (int)S2, N, c->st);
+      rc = accum_mfma_dev<R>(ring, S2 * N, (int)K, an_d, 0, (int)ms, c->states, (int)S2, N, c->st);
     hipEventRecord(ev[4], c->st);
     if (hipEventSynchronize(ev[4]) != hipSuccess) rc = SEP_ERR_HIP;
     if (it >= 0 && rc == SEP_OK) {

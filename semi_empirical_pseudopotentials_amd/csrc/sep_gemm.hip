@@ -215,6 +215,74 @@ template int combine_dev<double>(const double *, long, const double *, long, lon
 template int combine_dev<float>(const float *, long, const double *, long, long, long, float *,
                                 hipStream_t);
 
+// ------------------------------------------------ Newton accumulation GEMM
+// acc[s][ie][g] += sum_{k < cnt} anT[j0 + k][ie] * ring[k][s][g]
+// (filter/filter.c:74-78 for cnt stored Newton vectors at once) as a matrix
+// product on the matrix cores: targets x K times K x points.  Block: 256
+// threads, 32 targets x 256 points; wave w owns 64 points (4 MFMA column
+// tiles) x 32 targets (2 row tiles).  The K-block partial is formed in R
+// (FP64 for the FP64 filter, FP32 for the complex64 filter) and added to the
+// FP64 accumulators.
+template <typename R>
+__global__ void __launch_bounds__(256) k_accum_mfma(const R *__restrict__ ring, long slot, int cnt,
+                                                    const double *__restrict__ anT, long j0,
+                                                    int ms, double *__restrict__ acc, long N) {
+  const long s = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long g0 = (long)blockIdx.x * 256 + w * 64;
+  const int ie0 = blockIdx.z * 32;
+  const int kq = lane >> 4, li = lane & 15;
+  const R *rp = ring + s * N;
+  typename Mfma<R>::acc_t d[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) d[x][y] = typename Mfma<R>::acc_t{0, 0, 0, 0};
+#pragma unroll 4
+  for (int k = 0; k < cnt; k += 4) {
+    const int kk = k + kq;
+    const bool kin = kk < cnt;
+    R a[2], b[4];
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      const int ie = ie0 + 16 * x + li;
+      a[x] = (kin && ie < ms) ? (R)anT[(j0 + kk) * ms + ie] : (R)0;
+    }
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const long g = g0 + 16 * y + li;
+      b[y] = (kin && g < N) ? rp[(long)kk * slot + g] : (R)0;
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) d[x][y] = Mfma<R>::op(a[x], b[y], d[x][y]);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int ie = ie0 + 16 * x + Mfma<R>::row(lane, r);
+        const long g = g0 + 16 * y + li;
+        if (ie < ms && g < N) acc[(s * ms + ie) * N + g] += (double)d[x][y][r];
+      }
+}
+
+template <typename R>
+int accum_mfma_dev(const R *ring, long slot, int cnt, const double *anT_d, long j0, int ms,
+                   double *acc, int ns, long N, hipStream_t st) {
+  dim3 grid((unsigned)((N + 255) / 256), (unsigned)ns, (unsigned)((ms + 31) / 32));
+  hipLaunchKernelGGL(k_accum_mfma<R>, grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms, acc,
+                     N);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+template int accum_mfma_dev<double>(const double *, long, int, const double *, long, int, double *,
+                                    int, long, hipStream_t);
+template int accum_mfma_dev<float>(const float *, long, int, const double *, long, int, double *,
+                                   int, long, hipStream_t);
+
 // ---------------------------------------------------- double-double Gram
 // Each thread owns one (i, j) of a 16x16 tile and accumulates
 // sum_g A[i][g] A[j][g] over its K chunk as an unevaluated sum hi + lo

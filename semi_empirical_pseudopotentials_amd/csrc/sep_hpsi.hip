@@ -45,8 +45,7 @@ __device__ __forceinline__ void block_sum4(double s[4], double *part, long blk) 
 }
 
 template <typename R>
-__device__ __forceinline__ cx<R> epilogue(const Epi<R> &e, cx<R> t, cx<R> p, long gi, double s[4]) {
-  R V = e.pot[gi];
+__device__ __forceinline__ cx<R> epilogue(const Epi<R> &e, cx<R> t, cx<R> p, R V, double s[4]) {
   R a = (R)e.alpha, b = (R)e.beta;
   cx<R> o;
   o.re = a * (t.re + V * p.re) - b * p.re;
@@ -69,8 +68,8 @@ template <typename R, int NX, int T>
 __global__ void __launch_bounds__(T) k_rows_fwd(VecView<R> in, cx<R> *tb, long nrows, long N,
                                                 const cx<R> *twx) {
   using PL = FftPlan<NX>;
-  constexpr int G = PL::G, E = PL::E, ROWS = T / G, P = PL::P;
-  __shared__ cx<R> buf[ROWS][NX];
+  constexpr int G = PL::G, E = PL::E, ROWS = T / G, P = PL::P, PN = lds_line<NX>();
+  __shared__ cx<R> buf[ROWS][PN];
   __shared__ cx<R> tw[NX];
   load_tw<R, NX>(tw, twx);
   __syncthreads();
@@ -80,6 +79,8 @@ __global__ void __launch_bounds__(T) k_rows_fwd(VecView<R> in, cx<R> *tb, long n
   const long row = (long)blockIdx.x * ROWS + rl;
   const bool act = (rl < ROWS) && (row < nrows);
   const int rls = rl < ROWS ? rl : 0;
+  TwFor<NX, -1, R> tc;
+  tc.init(g, tw);
   cx<R> v[E];
   constexpr int r0 = Pass<NX, -1>::radix(0);
   if (act) {
@@ -88,7 +89,7 @@ __global__ void __launch_bounds__(T) k_rows_fwd(VecView<R> in, cx<R> *tb, long n
 #pragma unroll
       for (int k = 0; k < r0; k++) v[b * r0 + k] = in.ld(vec, row * NX + in_idx<NX, -1, 0>(g, b, k));
   }
-  run_passes<NX, -1, 0, P>(v, g, act, tw, [&](int i) -> cx<R> & { return buf[rls][i]; });
+  run_passes_c<NX, -1, 0, P>(v, g, act, tc, [&](int i) -> cx<R> & { return buf[rls][lds_pad(i)]; });
   constexpr int rl_ = Pass<NX, -1>::radix(P - 1);
   if (act) {
 #pragma unroll
@@ -104,8 +105,8 @@ template <typename R, int NX, int T>
 __global__ void __launch_bounds__(T) k_rows_bwd(const cx<R> *tb, VecView<R> pin, VecView<R> out,
                                                 Epi<R> epi, long nrows, long N, const cx<R> *twx) {
   using PL = FftPlan<NX>;
-  constexpr int G = PL::G, E = PL::E, ROWS = T / G, P = PL::P;
-  __shared__ cx<R> buf[ROWS][NX];
+  constexpr int G = PL::G, E = PL::E, ROWS = T / G, P = PL::P, PN = lds_line<NX>();
+  __shared__ cx<R> buf[ROWS][PN];
   __shared__ cx<R> tw[NX];
   load_tw<R, NX>(tw, twx);
   __syncthreads();
@@ -115,25 +116,36 @@ __global__ void __launch_bounds__(T) k_rows_bwd(const cx<R> *tb, VecView<R> pin,
   const long row = (long)blockIdx.x * ROWS + rl;
   const bool act = (rl < ROWS) && (row < nrows);
   const int rls = rl < ROWS ? rl : 0;
+  TwFor<NX, 1, R> tc;
+  tc.init(g, tw);
   cx<R> v[E];
   constexpr int r0 = Pass<NX, 1>::radix(0);
+  constexpr int rl_ = Pass<NX, 1>::radix(P - 1);
+  cx<R> p[E];
+  R V[E];
   if (act) {
 #pragma unroll
     for (int b = 0; b < E / r0; b++)
 #pragma unroll
       for (int k = 0; k < r0; k++) v[b * r0 + k] = tb[vec * N + row * NX + in_idx<NX, 1, 0>(g, b, k)];
+#pragma unroll
+    for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+      for (int k = 0; k < rl_; k++) {
+        const long gi = row * NX + out_idx<NX, 1, P - 1>(g, b, k);
+        p[b * rl_ + k] = pin.ld(vec, gi);
+        V[b * rl_ + k] = epi.pot[gi];
+      }
   }
-  run_passes<NX, 1, 0, P>(v, g, act, tw, [&](int i) -> cx<R> & { return buf[rls][i]; });
-  constexpr int rl_ = Pass<NX, 1>::radix(P - 1);
+  run_passes_c<NX, 1, 0, P>(v, g, act, tc, [&](int i) -> cx<R> & { return buf[rls][lds_pad(i)]; });
   double s[4] = {0.0, 0.0, 0.0, 0.0};
   if (act) {
 #pragma unroll
     for (int b = 0; b < E / rl_; b++)
 #pragma unroll
       for (int k = 0; k < rl_; k++) {
-        long gi = row * NX + out_idx<NX, 1, P - 1>(g, b, k);
-        cx<R> p = pin.ld(vec, gi);
-        out.st(vec, gi, epilogue(epi, v[b * rl_ + k], p, gi, s));
+        const long gi = row * NX + out_idx<NX, 1, P - 1>(g, b, k);
+        out.st(vec, gi, epilogue(epi, v[b * rl_ + k], p[b * rl_ + k], V[b * rl_ + k], s));
       }
   }
   if (epi.part) block_sum4<T>(s, epi.part, (long)blockIdx.y * gridDim.x + blockIdx.x);
@@ -239,76 +251,87 @@ struct PlaneCfg {
   static constexpr int LD = NX + 1;  // padded row pitch
 };
 
-// forward: view plane -> x FFT rows -> LDS -> y FFT columns -> Tbuf
+// forward: view plane -> x FFT rows -> LDS -> y FFT columns -> Tbuf.
+// All rows of the plane are loaded into registers before any compute, so the
+// whole plane is in flight at once (one block per CU: the plane fills LDS).
 template <typename R, int NX, int T>
 __global__ void __launch_bounds__(T) k_plane_fwd(VecView<R> in, cx<R> *tb, long N, int nz,
                                                  const cx<R> *twg) {
   using PL = FftPlan<NX>;
   constexpr int G = PL::G, E = PL::E, P = PL::P, LD = PlaneCfg<NX>::LD;
-  constexpr int RX = T / G;  // lines per round
+  constexpr int RX = T / G;                 // lines per round
+  constexpr int NR = (NX + RX - 1) / RX;    // rounds
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
   cx<R> *tw = buf + NX * LD;
-  load_tw<R, NX>(tw, twg);
-  __syncthreads();
   const long vec = blockIdx.x / nz;
   const long z = blockIdx.x % nz;
   const long pbase = z * (long)NX * NX;
   const int tl = threadIdx.x / G, g = threadIdx.x % G;
   constexpr int r0 = Pass<NX, -1>::radix(0);
   constexpr int rl_ = Pass<NX, -1>::radix(P - 1);
-  // ---- rows
-  for (int y0 = 0; y0 < NX; y0 += RX) {
-    const int row = y0 + tl;
-    const bool act = (tl < RX) && (row < NX);
-    const int rs = act ? row : 0;
-    cx<R> v[E];
-    if (act) {
+  cx<R> v[NR][E];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int row = r * RX + tl;
+    if ((tl < RX) && (row < NX)) {
 #pragma unroll
       for (int b = 0; b < E / r0; b++)
 #pragma unroll
-        for (int k = 0; k < r0; k++) v[b * r0 + k] = in.ld(vec, pbase + rs * NX + in_idx<NX, -1, 0>(g, b, k));
+        for (int k = 0; k < r0; k++)
+          v[r][b * r0 + k] = in.ld(vec, pbase + row * NX + in_idx<NX, -1, 0>(g, b, k));
     }
+  }
+  load_tw<R, NX>(tw, twg);
+  __syncthreads();
+  // ---- rows
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int row = r * RX + tl;
+    const bool act = (tl < RX) && (row < NX);
+    const int rs = act ? row : 0;
     auto at = [&](int i) -> cx<R> & { return buf[rs * LD + i]; };
-    run_passes<NX, -1, 0, P>(v, g, act, tw, at);
+    run_passes<NX, -1, 0, P>(v[r], g, act, tw, at);
     if constexpr (P > 1) __syncthreads();
-    write_out<NX, -1, P - 1>(v, g, act, at);
+    write_out<NX, -1, P - 1>(v[r], g, act, at);
   }
   __syncthreads();
   // ---- columns (consecutive threads -> consecutive columns)
   const int cl = threadIdx.x % RX, gc = threadIdx.x / RX;
-  for (int x0 = 0; x0 < NX; x0 += RX) {
-    const int col = x0 + cl;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int col = r * RX + cl;
     const bool act = (gc < G) && (col < NX);
     const int cs = act ? col : 0;
     const int gs = act ? gc : 0;
-    cx<R> v[E];
+    cx<R> *w = v[0];
     auto at = [&](int i) -> cx<R> & { return buf[i * LD + cs]; };
-    read_in<NX, -1, 0>(v, gs, act, at);
-    run_passes<NX, -1, 0, P>(v, gs, act, tw, at);
+    read_in<NX, -1, 0>(w, gs, act, at);
+    run_passes<NX, -1, 0, P>(w, gs, act, tw, at);
     if (act) {
 #pragma unroll
       for (int b = 0; b < E / rl_; b++)
 #pragma unroll
         for (int k = 0; k < rl_; k++)
-          tb[vec * N + pbase + (long)out_idx<NX, -1, P - 1>(gs, b, k) * NX + cs] = v[b * rl_ + k];
+          tb[vec * N + pbase + (long)out_idx<NX, -1, P - 1>(gs, b, k) * NX + cs] = w[b * rl_ + k];
     }
     __syncthreads();
   }
 }
 
-// backward + epilogue: Tbuf plane -> y FFT columns -> LDS -> x FFT rows -> out
+// backward + epilogue: Tbuf plane -> y FFT columns -> LDS -> x FFT rows -> out.
+// The column data are prefetched for all rounds; the epilogue operands of a
+// row round (psi_{j-1}, V) are loaded before that round's FFT work.
 template <typename R, int NX, int T>
 __global__ void __launch_bounds__(T) k_plane_bwd(const cx<R> *tb, VecView<R> pin, VecView<R> out,
                                                  Epi<R> epi, long N, int nz, const cx<R> *twg) {
   using PL = FftPlan<NX>;
   constexpr int G = PL::G, E = PL::E, P = PL::P, LD = PlaneCfg<NX>::LD;
   constexpr int RX = T / G;
+  constexpr int NR = (NX + RX - 1) / RX;
   extern __shared__ char smem_raw[];
   cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
   cx<R> *tw = buf + NX * LD;
-  load_tw<R, NX>(tw, twg);
-  __syncthreads();
   const long vec = blockIdx.x / nz;
   const long z = blockIdx.x % nz;
   const long pbase = z * (long)NX * NX;
@@ -316,44 +339,63 @@ __global__ void __launch_bounds__(T) k_plane_bwd(const cx<R> *tb, VecView<R> pin
   constexpr int rl_ = Pass<NX, 1>::radix(P - 1);
   // ---- columns
   const int cl = threadIdx.x % RX, gc = threadIdx.x / RX;
-  for (int x0 = 0; x0 < NX; x0 += RX) {
-    const int col = x0 + cl;
-    const bool act = (gc < G) && (col < NX);
-    const int cs = act ? col : 0;
-    const int gs = act ? gc : 0;
-    cx<R> v[E];
-    if (act) {
+  cx<R> v[NR][E];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int col = r * RX + cl;
+    if ((gc < G) && (col < NX)) {
 #pragma unroll
       for (int b = 0; b < E / r0; b++)
 #pragma unroll
         for (int k = 0; k < r0; k++)
-          v[b * r0 + k] = tb[vec * N + pbase + (long)in_idx<NX, 1, 0>(gs, b, k) * NX + cs];
+          v[r][b * r0 + k] = tb[vec * N + pbase + (long)in_idx<NX, 1, 0>(gc, b, k) * NX + col];
     }
+  }
+  load_tw<R, NX>(tw, twg);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int col = r * RX + cl;
+    const bool act = (gc < G) && (col < NX);
+    const int cs = act ? col : 0;
+    const int gs = act ? gc : 0;
     auto at = [&](int i) -> cx<R> & { return buf[i * LD + cs]; };
-    run_passes<NX, 1, 0, P>(v, gs, act, tw, at);
+    run_passes<NX, 1, 0, P>(v[r], gs, act, tw, at);
     if constexpr (P > 1) __syncthreads();
-    write_out<NX, 1, P - 1>(v, gs, act, at);
+    write_out<NX, 1, P - 1>(v[r], gs, act, at);
   }
   __syncthreads();
   // ---- rows + epilogue
   const int tl = threadIdx.x / G, g = threadIdx.x % G;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int y0 = 0; y0 < NX; y0 += RX) {
-    const int row = y0 + tl;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int row = r * RX + tl;
     const bool act = (tl < RX) && (row < NX);
     const int rs = act ? row : 0;
-    cx<R> v[E];
-    auto at = [&](int i) -> cx<R> & { return buf[rs * LD + i]; };
-    read_in<NX, 1, 0>(v, g, act, at);
-    run_passes<NX, 1, 0, P>(v, g, act, tw, at);
+    cx<R> p[E];
+    R V[E];
     if (act) {
 #pragma unroll
       for (int b = 0; b < E / rl_; b++)
 #pragma unroll
         for (int k = 0; k < rl_; k++) {
-          long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
-          cx<R> p = pin.ld(vec, gi);
-          out.st(vec, gi, epilogue(epi, v[b * rl_ + k], p, gi, s));
+          const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
+          p[b * rl_ + k] = pin.ld(vec, gi);
+          V[b * rl_ + k] = epi.pot[gi];
+        }
+    }
+    cx<R> *w = v[0];
+    auto at = [&](int i) -> cx<R> & { return buf[rs * LD + i]; };
+    read_in<NX, 1, 0>(w, g, act, at);
+    run_passes<NX, 1, 0, P>(w, g, act, tw, at);
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+        for (int k = 0; k < rl_; k++) {
+          const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
+          out.st(vec, gi, epilogue(epi, w[b * rl_ + k], p[b * rl_ + k], V[b * rl_ + k], s));
         }
     }
     __syncthreads();
@@ -361,13 +403,254 @@ __global__ void __launch_bounds__(T) k_plane_bwd(const cx<R> *tb, VecView<R> pin
   if (epi.part) block_sum4<T>(s, epi.part, blockIdx.x);
 }
 
+// ------------------------------------------- wave-local plane kernels
+// When the threads of one line fit a wavefront (64 % G == 0), every FFT
+// exchange of a row or a column stays inside one wavefront; only the
+// row <-> column transpose needs block barriers.  Line positions in LDS are
+// xor-swizzled (lds_swz) so the strided exchange patterns spread over the
+// banks; FP32 threads keep their twiddles in registers (TwFor).
+// If the plane does not fit LDS, the transpose goes through the buffer in
+// NH chunks of NX/NH columns while the plane stays in registers.
+// The spectrum of a plane is stored column-major, Tbuf[z][kx][ky], so that
+// the column phase writes (and the backward column phase reads) contiguous
+// ky runs.
+template <typename R, int NX>
+constexpr int plane_chunks() {
+  // smallest power of two (dividing NX) whose chunk buffer (plus the
+  // twiddle table) fits the 160 KB of LDS of a gfx950 workgroup
+  int nh = 1;
+  while (nh < NX && NX % (2 * nh) == 0 &&
+         sizeof(cx<R>) * ((size_t)NX * (NX / nh + 1) + NX) > 160 * 1024 - 1024)
+    nh *= 2;
+  return nh;
+}
+
+template <typename R, int NX, int T>
+__global__ void __launch_bounds__(T) k_plane_fwd_w(VecView<R> in, cx<R> *tb, long N, int nz,
+                                                   const cx<R> *twg) {
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, P = PL::P;
+  static_assert(64 % G == 0 && NX % 8 == 0, "lines must not straddle wavefronts");
+  constexpr int NH = plane_chunks<R, NX>(), W = NX / NH, LDc = W + 1;
+  constexpr int RX = T / G;                  // lines per round
+  constexpr int NR = (NX + RX - 1) / RX;     // row rounds
+  constexpr int NCR = (W + RX - 1) / RX;     // column rounds per chunk
+  constexpr int RS = NX + 1;                 // row-exchange slice pitch (NH > 1)
+  static_assert(NH == 1 || RX * RS <= NX * LDc, "row exchange slices exceed the chunk buffer");
+  extern __shared__ char smem_raw[];
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);  // [NX][LDc], row y at lds_swz(y)
+  cx<R> *tw = buf + NX * LDc;
+  const long vec = blockIdx.x / nz;
+  const long z = blockIdx.x % nz;
+  const long pbase = z * (long)NX * NX;
+  const int tl = threadIdx.x / G, g = threadIdx.x % G;
+  constexpr int r0 = Pass<NX, -1>::radix(0);
+  constexpr int rl_ = Pass<NX, -1>::radix(P - 1);
+  cx<R> v[NR][E];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int row = r * RX + tl;
+    if (row < NX) {
+#pragma unroll
+      for (int b = 0; b < E / r0; b++)
+#pragma unroll
+        for (int k = 0; k < r0; k++)
+          v[r][b * r0 + k] = in.ld(vec, pbase + row * NX + in_idx<NX, -1, 0>(g, b, k));
+    }
+  }
+  load_tw<R, NX>(tw, twg);
+  __syncthreads();
+  TwFor<NX, -1, R> tc;
+  tc.init(g, tw);
+  // ---- rows: x FFT; the results stay in registers.  Exchanges use the row's
+  // own buffer row (NH == 1) or slice tl of the chunk buffer.
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int row = r * RX + tl;
+    const bool act = row < NX;
+    const int base = (NH == 1 ? (act ? row : 0) * LDc : tl * RS);
+    auto at = [&](int i) -> cx<R> & { return buf[base + lds_swz(i)]; };
+    run_passes_c<NX, -1, 0, P, WaveSync>(v[r], g, act, tc, at);
+  }
+  // ---- columns, chunk by chunk
+#pragma unroll
+  for (int c = 0; c < NH; c++) {
+    __syncthreads();  // row exchanges / the previous chunk's columns are done
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int row = r * RX + tl;
+      if (row < NX) {
+#pragma unroll
+        for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+          for (int k = 0; k < rl_; k++) {
+            const int kx = out_idx<NX, -1, P - 1>(g, b, k);
+            if (kx / W == c) buf[lds_swz(row) * LDc + (kx - c * W)] = v[r][b * rl_ + k];
+          }
+      }
+    }
+    __syncthreads();  // the transpose
+#pragma unroll
+    for (int cr = 0; cr < NCR; cr++) {
+      const int cl = cr * RX + tl;
+      const bool act = cl < W;
+      const int cs = act ? cl : 0;
+      cx<R> w[E];
+      auto at = [&](int i) -> cx<R> & { return buf[lds_swz(i) * LDc + cs]; };
+      read_in<NX, -1, 0>(w, g, act, at);
+      run_passes_c<NX, -1, 0, P, WaveSync>(w, g, act, tc, at);
+      if (act) {
+        cx<R> *dst = tb + vec * N + pbase + (long)(c * W + cs) * NX;
+#pragma unroll
+        for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+          for (int k = 0; k < rl_; k++) dst[out_idx<NX, -1, P - 1>(g, b, k)] = w[b * rl_ + k];
+      }
+    }
+  }
+}
+
+template <typename R, int NX, int T>
+__global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> pin,
+                                                   VecView<R> out, Epi<R> epi, long N, int nz,
+                                                   const cx<R> *twg) {
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, P = PL::P;
+  static_assert(64 % G == 0 && NX % 8 == 0, "lines must not straddle wavefronts");
+  constexpr int NH = plane_chunks<R, NX>(), W = NX / NH, LDc = W + 1;
+  constexpr int RX = T / G;
+  constexpr int NR = (NX + RX - 1) / RX;
+  constexpr int NCR = (W + RX - 1) / RX;
+  constexpr int RS = NX + 1;
+  static_assert(NH == 1 || RX * RS <= NX * LDc, "row exchange slices exceed the chunk buffer");
+  extern __shared__ char smem_raw[];
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
+  cx<R> *tw = buf + NX * LDc;
+  const long vec = blockIdx.x / nz;
+  const long z = blockIdx.x % nz;
+  const long pbase = z * (long)NX * NX;
+  const int tl = threadIdx.x / G, g = threadIdx.x % G;
+  constexpr int r0 = Pass<NX, 1>::radix(0);
+  constexpr int rl_ = Pass<NX, 1>::radix(P - 1);
+  constexpr int NVR = NH == 1 ? 1 : NR;  // row inputs held in registers
+  cx<R> v[NVR][E];
+  TwFor<NX, 1, R> tc;
+#pragma unroll
+  for (int c = 0; c < NH; c++) {
+    // ---- columns of chunk c: load, y FFT, into the buffer
+    cx<R> w[NCR][E];
+#pragma unroll
+    for (int cr = 0; cr < NCR; cr++) {
+      const int cl = cr * RX + tl;
+      if (cl < W) {
+        const cx<R> *src = tb + vec * N + pbase + (long)(c * W + cl) * NX;
+#pragma unroll
+        for (int b = 0; b < E / r0; b++)
+#pragma unroll
+          for (int k = 0; k < r0; k++) w[cr][b * r0 + k] = src[in_idx<NX, 1, 0>(g, b, k)];
+      }
+    }
+    if (c == 0) {
+      load_tw<R, NX>(tw, twg);
+      __syncthreads();
+      tc.init(g, tw);
+    } else {
+      __syncthreads();  // the previous chunk's gather is done
+    }
+#pragma unroll
+    for (int cr = 0; cr < NCR; cr++) {
+      const int cl = cr * RX + tl;
+      const bool act = cl < W;
+      const int cs = act ? cl : 0;
+      auto at = [&](int i) -> cx<R> & { return buf[lds_swz(i) * LDc + cs]; };
+      run_passes_c<NX, 1, 0, P, WaveSync>(w[cr], g, act, tc, at);
+      WaveSync::sync();
+      write_out<NX, 1, P - 1>(w[cr], g, act, at);
+    }
+    __syncthreads();  // the transpose
+    if constexpr (NH > 1) {
+      // gather this chunk's x positions of every row this thread owns
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const int row = r * RX + tl;
+        if (row < NX) {
+#pragma unroll
+          for (int b = 0; b < E / r0; b++)
+#pragma unroll
+            for (int k = 0; k < r0; k++) {
+              const int x = in_idx<NX, 1, 0>(g, b, k);
+              if (x / W == c) v[r][b * r0 + k] = buf[lds_swz(row) * LDc + (x - c * W)];
+            }
+        }
+      }
+    }
+  }
+  if constexpr (NH > 1) __syncthreads();  // the buffer is free for the row exchanges
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int row = r * RX + tl;
+    const bool act = row < NX;
+    const int rs = act ? row : 0;
+    cx<R> p[E];
+    R V[E];
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+        for (int k = 0; k < rl_; k++) {
+          const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
+          p[b * rl_ + k] = pin.ld(vec, gi);
+          V[b * rl_ + k] = epi.pot[gi];
+        }
+    }
+    cx<R> *x = v[NH == 1 ? 0 : r];
+    if constexpr (NH == 1) {
+      // the row is in the buffer (row y at lds_swz(y), x unswizzled); the
+      // exchanges then run in place in that row with swizzled x
+      auto at0 = [&](int i) -> cx<R> & { return buf[lds_swz(rs) * LDc + i]; };
+      read_in<NX, 1, 0>(x, g, act, at0);
+      auto at = [&](int i) -> cx<R> & { return buf[lds_swz(rs) * LDc + lds_swz(i)]; };
+      run_passes_c<NX, 1, 0, P, WaveSync>(x, g, act, tc, at);
+    } else {
+      auto at = [&](int i) -> cx<R> & { return buf[tl * RS + lds_swz(i)]; };
+      run_passes_c<NX, 1, 0, P, WaveSync>(x, g, act, tc, at);
+    }
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+        for (int k = 0; k < rl_; k++) {
+          const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
+          out.st(vec, gi, epilogue(epi, x[b * rl_ + k], p[b * rl_ + k], V[b * rl_ + k], s));
+        }
+    }
+  }
+  if (epi.part) block_sum4<T>(s, epi.part, blockIdx.x);
+}
+
 // ================================================================= dispatch
+template <int n>
+constexpr bool plane_colmajor() {
+  return 64 % FftPlan<n>::G == 0 && n % 8 == 0;
+}
 template <typename R, int NX>
 constexpr size_t plane_lds() {
+  if constexpr (plane_colmajor<NX>())
+    return sizeof(cx<R>) * ((size_t)NX * (NX / plane_chunks<R, NX>() + 1) + NX);
   return sizeof(cx<R>) * ((size_t)NX * PlaneCfg<NX>::LD + NX);
 }
 template <typename R, int NX>
 constexpr int plane_threads() {
+  if constexpr (plane_colmajor<NX>()) {
+    // lines per round limited so that the row-exchange slices fit the chunk buffer
+    constexpr int G = FftPlan<NX>::G;
+    constexpr int LDc = NX / plane_chunks<R, NX>() + 1;
+    int t = NX >= 64 ? 512 : 256;
+    if (plane_chunks<R, NX>() > 1)
+      while (t > 64 && (t / G) * (NX + 1) > NX * LDc) t -= 64;
+    return t;
+  }
   return NX >= 64 ? 512 : 256;
 }
 
@@ -468,8 +751,12 @@ bool HOps<R>::pencil(int nz, cx<R> *tb, long N, int nx, int ny, int nvec, const 
   return false;
 }
 
-#define SEP_PLANE_SIZES_F32(X) X(16) X(24) X(32) X(36) X(48) X(64) X(80) X(96) X(128)
-#define SEP_PLANE_SIZES_F64(X) X(16) X(24) X(32) X(36) X(48) X(64) X(80) X(96)
+// Plane-kernel lengths per precision: those whose kernels fit the register
+// file without spilling (80 needs 20 points per thread; FP64 128 would hold
+// a half plane of 64-bit values per thread).  Other lengths take the
+// row + strided-line path.
+#define SEP_PLANE_SIZES_F32(X) X(16) X(24) X(32) X(36) X(48) X(64) X(96) X(128)
+#define SEP_PLANE_SIZES_F64(X) X(16) X(24) X(32) X(36) X(48) X(64) X(96)
 
 template <typename R, int n>
 static void plane_fwd_launch(VecView<R> in, cx<R> *tb, long N, int nz, int nvec, const cx<R> *tw,
@@ -477,13 +764,23 @@ static void plane_fwd_launch(VecView<R> in, cx<R> *tb, long N, int nz, int nvec,
   constexpr int T = plane_threads<R, n>();
   constexpr size_t lds = plane_lds<R, n>();
   static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_plane_fwd<R, n, T>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
+  if constexpr (plane_colmajor<n>()) {
+    if (!attr) {
+      hipFuncSetAttribute((const void *)k_plane_fwd_w<R, n, T>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_plane_fwd_w<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, in,
+                       tb, N, nz, tw);
+  } else {
+    if (!attr) {
+      hipFuncSetAttribute((const void *)k_plane_fwd<R, n, T>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_plane_fwd<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, in,
+                       tb, N, nz, tw);
   }
-  hipLaunchKernelGGL((k_plane_fwd<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, in, tb,
-                     N, nz, tw);
 }
 template <typename R, int n>
 static void plane_bwd_launch(const cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi, long N,
@@ -491,13 +788,23 @@ static void plane_bwd_launch(const cx<R> *tb, VecView<R> pin, VecView<R> out, Ep
   constexpr int T = plane_threads<R, n>();
   constexpr size_t lds = plane_lds<R, n>();
   static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_plane_bwd<R, n, T>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
+  if constexpr (plane_colmajor<n>()) {
+    if (!attr) {
+      hipFuncSetAttribute((const void *)k_plane_bwd_w<R, n, T>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_plane_bwd_w<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st,
+                       tb, pin, out, epi, N, nz, tw);
+  } else {
+    if (!attr) {
+      hipFuncSetAttribute((const void *)k_plane_bwd<R, n, T>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_plane_bwd<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, tb,
+                       pin, out, epi, N, nz, tw);
   }
-  hipLaunchKernelGGL((k_plane_bwd<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, tb, pin,
-                     out, epi, N, nz, tw);
 }
 
 template <>
@@ -558,6 +865,18 @@ bool HOps<double>::plane_bwd(int nxy, const cx<double> *tb, VecView<double> pin,
 }
 
 // ----------------------------------------------------------------- public
+// Plane kernels of these lengths store Tbuf column-major ([kx][ky]).
+static bool plane_is_colmajor(int n) {
+  switch (n) {
+#define C(x) \
+  case x:    \
+    return plane_colmajor<x>();
+    SEP_LINE_SIZES(C)
+#undef C
+  }
+  return false;
+}
+
 bool hop_supported(int n) {
   switch (n) {
 #define C(x) case x:
@@ -608,8 +927,10 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
     if (!HOps<R>::lines(ny, -1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st))
       return SEP_ERR_UNSUPPORTED;
   }
-  // pass B
-  if (!HOps<R>::pencil(nz, tb, N, nx, ny, nvec, twz, gd.kxx, gd.kyy, gd.kzz, gd.ekin, 1.0, st))
+  // pass B (a column-major plane spectrum swaps the roles of x and y)
+  const bool cm = plane && plane_is_colmajor(nx);
+  if (!HOps<R>::pencil(nz, tb, N, cm ? ny : nx, cm ? nx : ny, nvec, twz, cm ? gd.kyy : gd.kxx,
+                       cm ? gd.kxx : gd.kyy, gd.kzz, gd.ekin, 1.0, st))
     return SEP_ERR_UNSUPPORTED;
   // pass C
   Epi<R> epi{pot, alpha, beta, part};
@@ -632,17 +953,34 @@ int hpsi_apply_timed(const GridDev &gd, const VecView<R> &in, const VecView<R> &
                      const R *pot, int nvec, double alpha, double beta, hipStream_t st,
                      hipEvent_t *ev) {
   const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
-  if (gd.force_lines || !plane_supported(gd.nx, gd.ny, prec)) return SEP_ERR_UNSUPPORTED;
+  const bool plane = !gd.force_lines && plane_supported(gd.nx, gd.ny, prec);
   const cx<R> *twx = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
+  const cx<R> *twy = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[1] : gd.tw32[1]);
   const cx<R> *twz = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[2] : gd.tw32[2]);
+  const int nx = gd.nx, ny = gd.ny, nz = gd.nz;
+  const long N = gd.N;
   Epi<R> epi{pot, alpha, beta, nullptr};
+  bool ok = true;
   hipEventRecord(ev[0], st);
-  HOps<R>::plane_fwd(gd.nx, in, tb, gd.N, gd.nz, nvec, twx, st);
+  if (plane) {
+    ok &= HOps<R>::plane_fwd(nx, in, tb, N, nz, nvec, twx, st);
+  } else {
+    ok &= HOps<R>::rows_fwd(nx, in, tb, (long)ny * nz, N, nvec, twx, st);
+    ok &= HOps<R>::lines(ny, -1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st);
+  }
   hipEventRecord(ev[1], st);
-  HOps<R>::pencil(gd.nz, tb, gd.N, gd.nx, gd.ny, nvec, twz, gd.kxx, gd.kyy, gd.kzz, gd.ekin, 1.0, st);
+  const bool cm = plane && plane_is_colmajor(nx);
+  ok &= HOps<R>::pencil(nz, tb, N, cm ? ny : nx, cm ? nx : ny, nvec, twz, cm ? gd.kyy : gd.kxx,
+                        cm ? gd.kxx : gd.kyy, gd.kzz, gd.ekin, 1.0, st);
   hipEventRecord(ev[2], st);
-  HOps<R>::plane_bwd(gd.nx, tb, in, out, epi, gd.N, gd.nz, nvec, twx, st, nullptr);
+  if (plane) {
+    ok &= HOps<R>::plane_bwd(nx, tb, in, out, epi, N, nz, nvec, twx, st, nullptr);
+  } else {
+    ok &= HOps<R>::lines(ny, 1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st);
+    ok &= HOps<R>::rows_bwd(nx, tb, in, out, epi, (long)ny * nz, N, nvec, twx, st, nullptr);
+  }
   hipEventRecord(ev[3], st);
+  if (!ok) return SEP_ERR_UNSUPPORTED;
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 template int hpsi_apply_timed<double>(const GridDev &, const VecView<double> &,

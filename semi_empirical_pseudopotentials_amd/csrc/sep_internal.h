@@ -47,10 +47,15 @@ int dots_dev_f(const float *a, long lda, const float *b, long ldb, long m, long 
 int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStream_t st);
 int convert_d2f(const double *in, float *out, long n, hipStream_t st);
 int convert_f2d(const float *in, double *out, long n, hipStream_t st);
-// acc[(s*ms + ie)*N + g] += sum_{k<cnt} an[ie*nc + j0 + k] * ring[k*slot + s*N + g]
+// acc[(s*ms + ie)*N + g] += sum_{k<cnt} anT[(j0 + k)*ms + ie] * ring[k*slot + s*N + g]
 template <typename R>
-int accum_dev(const R *ring, long slot, int cnt, const double *an_d, long nc, long j0, int ms,
+int accum_dev(const R *ring, long slot, int cnt, const double *anT_d, long j0, int ms,
               double *acc, int ns, long N, hipStream_t st);
+
+// same contraction on the matrix cores (sep_gemm.hip)
+template <typename R>
+int accum_mfma_dev(const R *ring, long slot, int cnt, const double *anT_d, long j0, int ms,
+                   double *acc, int ns, long N, hipStream_t st);
 
 // ---- GEMMs (sep_gemm.hip); all operands state-major (row i = one state)
 // C[i][j] = sum_g A[i][g] B[j][g]   (m x mb), deterministic split-K

@@ -161,19 +161,15 @@ int convert_f2d(const float *in, double *out, long n, hipStream_t st) {
 
 // --------------------------------------------- K-blocked Newton accumulation
 // filter/filter.c:74-78 applied to `cnt` stored Newton vectors at once:
-//   acc[s][ie][g] += sum_k an[ie][j0+k] * ring[k][s][g]
-// The k sum is done in the reference's order (ascending j) directly into the
-// accumulator.  Targets are processed in register chunks of MSB.
+//   acc[s][ie][g] += sum_k anT[j0+k][ie] * ring[k][s][g]
+// The k sum runs in the reference's order (ascending j) directly into the
+// accumulator.  Targets are processed in register chunks of MSB; the
+// coefficients (transposed to [j][ie]) are read with uniform addresses,
+// i.e. through the scalar cache.
 template <typename R, int MSB>
 __global__ void __launch_bounds__(256) k_accum(const R *__restrict__ ring, long slot, int cnt,
-                                               const double *__restrict__ an, long nc, long j0,
-                                               int ms, double *__restrict__ acc, long N) {
-  extern __shared__ double a_s[];  // [ms][cnt]
-  for (int t = threadIdx.x; t < ms * cnt; t += blockDim.x) {
-    int ie = t / cnt, k = t % cnt;
-    a_s[t] = an[(long)ie * nc + j0 + k];
-  }
-  __syncthreads();
+                                               const double *__restrict__ anT, long j0, int ms,
+                                               double *__restrict__ acc, long N) {
   const long s = blockIdx.y;
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= N) return;
@@ -183,10 +179,23 @@ __global__ void __launch_bounds__(256) k_accum(const R *__restrict__ ring, long 
     double *ap = acc + (s * ms + i0) * N + g;
 #pragma unroll
     for (int q = 0; q < MSB; q++) a[q] = (i0 + q < ms) ? ap[(long)q * N] : 0.0;
-    for (int k = 0; k < cnt; k++) {
+    const double *cp = anT + j0 * ms + i0;
+    int k = 0;
+    for (; k + 8 <= cnt; k += 8) {  // 8 Newton vectors in flight per thread
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) x[u] = (double)rp[(long)(k + u) * slot];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+#pragma unroll
+        for (int q = 0; q < MSB; q++)
+          if (i0 + q < ms) a[q] += cp[(long)(k + u) * ms + q] * x[u];
+    }
+    for (; k < cnt; k++) {
       const double x = (double)rp[(long)k * slot];
 #pragma unroll
-      for (int q = 0; q < MSB; q++) a[q] += a_s[(i0 + q < ms ? i0 + q : 0) * cnt + k] * x;
+      for (int q = 0; q < MSB; q++)
+        if (i0 + q < ms) a[q] += cp[(long)k * ms + q] * x;
     }
 #pragma unroll
     for (int q = 0; q < MSB; q++)
@@ -195,18 +204,20 @@ __global__ void __launch_bounds__(256) k_accum(const R *__restrict__ ring, long 
 }
 
 template <typename R>
-int accum_dev(const R *ring, long slot, int cnt, const double *an_d, long nc, long j0, int ms,
+int accum_dev(const R *ring, long slot, int cnt, const double *anT_d, long j0, int ms,
               double *acc, int ns, long N, hipStream_t st) {
   dim3 grid((unsigned)((N + 255) / 256), (unsigned)ns);
-  size_t lds = sizeof(double) * (size_t)ms * cnt;
-  if (lds > 64 * 1024) return SEP_ERR_ARG;
-  hipLaunchKernelGGL((k_accum<R, 16>), grid, dim3(256), lds, st, ring, slot, cnt, an_d, nc, j0, ms,
-                     acc, N);
+  if (ms <= 16)
+    hipLaunchKernelGGL((k_accum<R, 16>), grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms,
+                       acc, N);
+  else
+    hipLaunchKernelGGL((k_accum<R, 32>), grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms,
+                       acc, N);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
-template int accum_dev<double>(const double *, long, int, const double *, long, long, int,
-                               double *, int, long, hipStream_t);
-template int accum_dev<float>(const float *, long, int, const double *, long, long, int, double *,
-                              int, long, hipStream_t);
+template int accum_dev<double>(const double *, long, int, const double *, long, int, double *,
+                               int, long, hipStream_t);
+template int accum_dev<float>(const float *, long, int, const double *, long, int, double *, int,
+                              long, hipStream_t);
 
 }  // namespace sep
