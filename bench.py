@@ -43,11 +43,29 @@ def parse():
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=24)
+    p.add_argument("--kernels-only", action="store_true",
+                   help="only launch the H.psi passes (FP32, 5 times) and a 256 MiB device "
+                        "copy that calibrates the HBM byte counters; for rocprofv3 --pmc runs")
     return p.parse_args()
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hpsi_fp32_128_traffic.md")
+
+
+def measured_traffic(kernel, n, nvec):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor),
+    or None when the summary does not cover this workload."""
+    if n != 128 or nvec != 8 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch") and kernel in line:
+            return int(line.split()[-1])
+    return None
 
 
 def main():
@@ -93,60 +111,76 @@ def main():
 
     # ------------------------------------------------ per-pass kernel timing
     nvec = (ns + 1) // 2
-    kern = {}
-    for prec, name in ((sep.SEP_PREC_FP32, "fp32"), (sep.SEP_PREC_FP64, "fp64")):
-        if name == "fp64" and args.no_fp64:
-            continue
-        kern[name] = ctx.bench_hpsi(prec, nvec, ms, 20)
-        log(f"{name} pass ms: A {kern[name][0]:.4f} B {kern[name][1]:.4f} "
-            f"C {kern[name][2]:.4f} accum(K=64) {kern[name][3]:.4f} A+B+C {kern[name][4]:.4f}")
-
+    if args.kernels_only:
+        k = ctx.bench_hpsi(sep.SEP_PREC_FP32, nvec, ms, 5)
+        a = torch.empty(64 << 20, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.add(a, 1.0, out=b)  # plain copy kernel: 256 MiB read + 256 MiB written
+        torch.cuda.synchronize()
+        print(json.dumps({"kernels_only": True, "pass_ms": list(k), "copy_bytes": 2 * a.numel() * 4}),
+              flush=True)
+        ctx.close()
+        return
     # ------------------------------------------------------------ timed loop
+    # The library samples the three H.psi passes of one Newton step per ring
+    # block, and that block's accumulation, with HIP events on its own stream
+    # while the timed filter runs (sep_set_timing).
     def run(prec):
         for _ in range(args.warmup):
             ctx.filter_dev(psi0_dev.data_ptr(), ns, an, zn, vmin, dE, prec)
         torch.cuda.synchronize()
+        ctx.set_timing(True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             ctx.filter_dev(psi0_dev.data_ptr(), ns, an, zn, vmin, dE, prec)
         torch.cuda.synchronize()
-        return time.perf_counter() - t0
+        t = time.perf_counter() - t0
+        live, samples = ctx.get_timing()
+        ctx.set_timing(False)
+        return t, live, samples
 
-    t32 = run(sep.SEP_PREC_FP32)
+    t32, live32, smp32 = run(sep.SEP_PREC_FP32)
     hpsi = ns * (nc - 1) * args.steps
     v32 = hpsi / t32
-    log(f"fp32 filter: {t32 / args.steps:.3f} s/step, {v32:.0f} H.psi/s")
-    v64 = None
+    log(f"fp32 filter: {t32 / args.steps:.3f} s/step, {v32:.0f} H.psi/s; live kernel ms "
+        f"A {live32[0]:.4f} B {live32[1]:.4f} C {live32[2]:.4f} accum {live32[3]:.3f} "
+        f"(samples {smp32})")
+    v64 = live64 = None
     if not args.no_fp64:
-        t64 = run(sep.SEP_PREC_FP64)
+        t64, live64, smp64 = run(sep.SEP_PREC_FP64)
         v64 = hpsi / t64
-        log(f"fp64 filter: {t64 / args.steps:.3f} s/step, {v64:.0f} H.psi/s")
+        log(f"fp64 filter: {t64 / args.steps:.3f} s/step, {v64:.0f} H.psi/s; live kernel ms "
+            f"A {live64[0]:.4f} B {live64[1]:.4f} C {live64[2]:.4f} accum {live64[3]:.3f}")
 
     # ------------------------------------------------------------- roofline
-    # algorithmic bytes per launch (nvec complex vectors of N points):
+    # Algorithmic bytes per launch, nvec complex vectors of N points
+    # (two real states per vector), c = complex bytes, r = real bytes:
     #   pass A: read psi (re, im planes) + write T          2c N nvec
     #   pass B: read T + write T                             2c N nvec
     #   pass C: read T, psi, V + write psi_j                 (3c + r) N nvec
-    # c = complex bytes, r = real bytes (V).
-    def roof(name, c, r):
-        k = kern[name]
+    # i.e. N (7c + r) per complex vector and H application.
+    def roof(live, c, r):
         per = {"A": 2 * c * N * nvec, "B": 2 * c * N * nvec, "C": (3 * c + r) * N * nvec}
         out = {}
         for i, key in enumerate("ABC"):
-            gbs = per[key] / (k[i] * 1e-3) / 1e9
-            out[key] = {"ms": k[i], "bytes": per[key], "GBs": gbs, "frac": gbs / PEAK_HBM_GBS}
-        tot = sum(per.values())
-        out["ABC"] = {"ms": k[4], "bytes": tot, "GBs": tot / (k[4] * 1e-3) / 1e9,
-                      "frac": tot / (k[4] * 1e-3) / 1e9 / PEAK_HBM_GBS}
+            gbs = per[key] / (live[i] * 1e-3) / 1e9
+            out[key] = {"ms": round(live[i], 5), "bytes": per[key], "GBs": round(gbs, 1),
+                        "frac": round(gbs / PEAK_HBM_GBS, 4)}
+        tot, tms = sum(per.values()), sum(live[:3])
+        out["ABC"] = {"ms": round(tms, 5), "bytes": tot, "GBs": round(tot / (tms * 1e-3) / 1e9, 1),
+                      "frac": round(tot / (tms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
         return out
 
-    r32 = roof("fp32", 8, 4)
+    r32 = roof(live32, 8, 4)
     dom = max("ABC", key=lambda q: r32[q]["ms"])
-    kname = {"A": "k_plane_fwd", "B": "k_pencil", "C": "k_plane_bwd"}[dom]
-    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(r32[dom]["GBs"], 1),
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(r32[dom]["frac"], 4),
-                "traffic": None,
-                "hpsi_passes_frac": round(r32["ABC"]["frac"], 4)}
+    kname = {"A": "k_plane_fwd_w<float,128>", "B": "k_pencil<float,128>",
+             "C": "k_plane_bwd_w<float,128>"}[dom]
+    traffic = measured_traffic(kname, n, nvec)
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": r32[dom]["GBs"],
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": r32[dom]["frac"],
+                "traffic": traffic, "algorithmic_bytes": r32[dom]["bytes"],
+                "hpsi_passes_frac": r32["ABC"]["frac"]}
 
     # ---------------------------------------------------------- CPU baseline
     cpu = None
@@ -191,7 +225,9 @@ def main():
         "complex_hpsi_per_s": round(v32 / 2, 1),
         "roofline": roofline,
         "roofline_passes_fp32": r32,
-        "roofline_passes_fp64": roof("fp64", 16, 8) if "fp64" in kern else None,
+        "roofline_passes_fp64": roof(live64, 16, 8) if live64 is not None else None,
+        "accum_ms_per_block": {"fp32": round(live32[3], 4),
+                               "fp64": round(live64[3], 4) if live64 is not None else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -155,6 +155,17 @@ int sep_rotate_dev(sep_ctx *ctx, int precision, const void *A_dev, long m, const
 int sep_bench_hpsi(sep_ctx *ctx, int precision, long nvec, long ms, long iters,
                    double *kernel_ms);
 
+/* Live kernel timing of the filter loop.  While on, sep_filter/sep_filter_dev
+ * record HIP events on the context's stream around the three H.psi passes of
+ * one Newton step per ring block, and around that block's accumulation
+ * launch (one host wait per block).  sep_set_timing resets the sums;
+ * sep_get_timing returns avg_ms[0..3] = passes A, B, C and the accumulation
+ * of a full ring block, and samples[0..1] = the number of sampled steps and
+ * of sampled full blocks.  No counterpart in the reference (it only prints
+ * clock() totals, filter/main.c:63-66,91-100). */
+int sep_set_timing(sep_ctx *ctx, int on);
+int sep_get_timing(sep_ctx *ctx, double *avg_ms, long *samples);
+
 /* -------------------------------------------------- resident state block */
 /* Upload/download m states (m x N FP64) to/from the context. */
 int sep_states_upload(sep_ctx *ctx, const double *psi, long m);

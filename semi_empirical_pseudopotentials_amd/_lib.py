@@ -85,6 +85,8 @@ SIGNATURES = {
     "sep_sigma": (_I, [_V, _L, _dp]),
     "sep_filter_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D, _I]),
     "sep_bench_hpsi": (_I, [_V, _I, _L, _L, _L, _dp]),
+    "sep_set_timing": (_I, [_V, _I]),
+    "sep_get_timing": (_I, [_V, _dp, ctypes.POINTER(ctypes.c_long)]),
     "sep_gram_dev": (_I, [_V, _I, _V, _L, _V, _L, _dp]),
     "sep_rotate_dev": (_I, [_V, _I, _V, _L, _dp, _L, _V]),
     "sep_deck_read": (_I, [ctypes.c_char_p, _V, ctypes.c_char_p, _I]),
@@ -330,6 +332,17 @@ class Context:
         rc = lib().sep_bench_hpsi(self._h, precision, nvec, ms, iters, _p(out))
         self._chk(rc, "sep_bench_hpsi")
         return out
+
+    def set_timing(self, on):
+        """Sample kernel durations inside the filter loop (see sep_set_timing)."""
+        self._chk(lib().sep_set_timing(self._h, 1 if on else 0), "sep_set_timing")
+
+    def get_timing(self):
+        """(avg ms of passes A, B, C, avg ms of one full-block accumulation), samples."""
+        out = np.zeros(4)
+        n = (ctypes.c_long * 2)()
+        self._chk(lib().sep_get_timing(self._h, _p(out), n), "sep_get_timing")
+        return out, (n[0], n[1])
 
     def gram_dev(self, A_ptr, m, B_ptr, mb, precision=SEP_PREC_FP64):
         C = np.empty((m, mb))

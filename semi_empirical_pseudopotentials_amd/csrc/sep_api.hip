@@ -33,6 +33,12 @@ struct sep_ctx {
   size_t scr_len = 0;
   void *ring = nullptr;
   size_t ring_bytes = 0;
+  // live kernel timing of the filter loop (sep_set_timing): one sampled
+  // Newton step per ring block (passes A, B, C) and that block's accumulation
+  bool timing = false;
+  hipEvent_t tev[6] = {};
+  double tsum[4] = {0, 0, 0, 0};
+  long tcount = 0, tacc_count = 0;
   std::string err;
 };
 
@@ -185,6 +191,8 @@ int sep_close(sep_ctx *c) {
                   (void *)c->pot32, (void *)c->states, (void *)c->states2, c->tb, (void *)c->scr,
                   c->ring})
     if (p) hipFree(p);
+  for (hipEvent_t e : c->tev)
+    if (e) hipEventDestroy(e);
   if (c->st) hipStreamDestroy(c->st);
   delete c;
   return SEP_OK;
@@ -293,7 +301,8 @@ const float *pot_of<float>(sep_ctx *c) {
 
 template <typename R>
 static int apply_views(sep_ctx *c, const VecView<R> &in, const VecView<R> &out, int nvec,
-                       double alpha, double beta, std::vector<double> *sums) {
+                       double alpha, double beta, std::vector<double> *sums,
+                       hipEvent_t *ev = nullptr) {
   if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
   const long N = c->gd.N;
   RCHK(ensure(c, &c->tb, &c->tb_bytes, sizeof(cx<R>) * (size_t)nvec * N));
@@ -306,7 +315,7 @@ static int apply_views(sep_ctx *c, const VecView<R> &in, const VecView<R> &out, 
   }
   long nblk = 0;
   int rc = hpsi_apply<R>(c->gd, in, out, reinterpret_cast<cx<R> *>(c->tb), pot_of<R>(c), nvec,
-                         alpha, beta, part, &nblk, c->st);
+                         alpha, beta, part, &nblk, c->st, ev);
   if (rc != SEP_OK) return fail(c, rc, "H.psi launch failed (grid %dx%dx%d)", c->gd.nx, c->gd.ny, c->gd.nz);
   if (sums) {
     std::vector<double> h((size_t)nblk * 4);
@@ -564,16 +573,40 @@ static int filter_impl(sep_ctx *c, const double *psi0, bool psi0_on_device, long
   HIPCHK(c, hipMalloc(&an_d, sizeof(double) * ms * nc));
   HIPCHK(c, hipMemcpyAsync(an_d, anT.data(), sizeof(double) * ms * nc, hipMemcpyHostToDevice, c->st));
   int rc = SEP_OK;
+  bool sampled = false;  // a step of the current ring block carries timing events
   for (long j = 1; j < nc && rc == SEP_OK; j++) {
     const long sp = (j - 1) % K, sj = j % K;
     VecView<R> in{ring + sp * slot, ring + sp * slot + N, nullptr, 2 * N};
     VecView<R> out{ring + sj * slot, ring + sj * slot + N, nullptr, 2 * N};
     // hnorm.c:19-24: psi_j = dE_1 H psi_{j-1} - (2 + z_{j-1} + Vmin dE_1) psi_{j-1}
     const double beta = 2.0 + zn[j - 1] + vmin * dE_1;
-    rc = apply_views<R>(c, in, out, (int)nvec, dE_1, beta, nullptr);
-    if (rc == SEP_OK && (sj == K - 1 || j == nc - 1))
+    const bool sample = c->timing && sj == K / 2;
+    rc = apply_views<R>(c, in, out, (int)nvec, dE_1, beta, nullptr, sample ? c->tev : nullptr);
+    sampled |= sample;
+    if (rc == SEP_OK && (sj == K - 1 || j == nc - 1)) {
+      if (sampled) hipEventRecord(c->tev[4], c->st);
       rc = accum_mfma_dev<R>(ring, slot, (int)(sj + 1), an_d, j - sj, (int)ms, c->states, (int)ns, N,
                         c->st);
+      if (sampled && rc == SEP_OK) {
+        hipEventRecord(c->tev[5], c->st);
+        if (hipEventSynchronize(c->tev[5]) != hipSuccess) {
+          rc = SEP_ERR_HIP;
+          break;
+        }
+        float t;
+        for (int q = 0; q < 3; q++) {
+          hipEventElapsedTime(&t, c->tev[q], c->tev[q + 1]);
+          c->tsum[q] += t;
+        }
+        c->tcount++;
+        if (sj + 1 == K) {  // only full blocks time the accumulation of K vectors
+          hipEventElapsedTime(&t, c->tev[4], c->tev[5]);
+          c->tsum[3] += t;
+          c->tacc_count++;
+        }
+      }
+      sampled = false;
+    }
   }
   if (rc == SEP_OK && nc == 1) rc = accum_mfma_dev<R>(ring, slot, 1, an_d, 0, (int)ms, c->states, (int)ns, N, c->st);
   hipStreamSynchronize(c->st);  // anT must outlive the async copy
@@ -757,7 +790,8 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
   double acc[5] = {0, 0, 0, 0, 0};
   int rc = SEP_OK;
   for (long it = -1; it < iters && rc == SEP_OK; it++) {  // it = -1: warm-up
-    rc = hpsi_apply_timed<R>(c->gd, in, out, tb, pot_of<R>(c), (int)nvec, 0.3, 1.7, c->st, ev);
+    rc = hpsi_apply<R>(c->gd, in, out, tb, pot_of<R>(c), (int)nvec, 0.3, 1.7, nullptr, nullptr,
+                       c->st, ev);
     if (rc == SEP_OK)
       rc = accum_mfma_dev<R>(ring, S2 * N, (int)K, an_d, 0, (int)ms, c->states, (int)S2, N, c->st);
     hipEventRecord(ev[4], c->st);
@@ -787,6 +821,28 @@ extern "C" int sep_bench_hpsi(sep_ctx *c, int precision, long nvec, long ms, lon
   if (precision == SEP_PREC_FP64) return bench_impl<double>(c, nvec, ms, iters, kernel_ms);
   if (precision == SEP_PREC_FP32) return bench_impl<float>(c, nvec, ms, iters, kernel_ms);
   return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
+}
+
+extern "C" int sep_set_timing(sep_ctx *c, int on) {
+  if (!c) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (on && !c->tev[0])
+    for (int i = 0; i < 6; i++) HIPCHK(c, hipEventCreate(&c->tev[i]));
+  c->timing = on != 0;
+  for (int q = 0; q < 4; q++) c->tsum[q] = 0.0;
+  c->tcount = c->tacc_count = 0;
+  return SEP_OK;
+}
+
+extern "C" int sep_get_timing(sep_ctx *c, double *avg_ms, long *samples) {
+  if (!c || !avg_ms) return SEP_ERR_ARG;
+  for (int q = 0; q < 3; q++) avg_ms[q] = c->tcount ? c->tsum[q] / (double)c->tcount : 0.0;
+  avg_ms[3] = c->tacc_count ? c->tsum[3] / (double)c->tacc_count : 0.0;
+  if (samples) {
+    samples[0] = c->tcount;
+    samples[1] = c->tacc_count;
+  }
+  return SEP_OK;
 }
 
 // ------------------------------------------------- dense device contractions

@@ -907,11 +907,12 @@ bool plane_supported(int nx, int ny, int prec) {
   return false;
 }
 
-// One H application (see file header) on nvec complex vectors.
+// One H application (see file header) on nvec complex vectors.  With ev,
+// events are recorded before pass A and after passes A, B and C (ev[0..3]).
 template <typename R>
 int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
                const R *pot, int nvec, double alpha, double beta, double *part, long *nblk,
-               hipStream_t st) {
+               hipStream_t st, hipEvent_t *ev) {
   const int nx = gd.nx, ny = gd.ny, nz = gd.nz;
   const long N = gd.N;
   const cx<R> *twx = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
@@ -919,6 +920,7 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
   const cx<R> *twz = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[2] : gd.tw32[2]);
   const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
   const bool plane = gd.force_lines ? false : plane_supported(nx, ny, prec);
+  if (ev) hipEventRecord(ev[0], st);
   // pass A
   if (plane) {
     if (!HOps<R>::plane_fwd(nx, in, tb, N, nz, nvec, twx, st)) return SEP_ERR_UNSUPPORTED;
@@ -927,11 +929,13 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
     if (!HOps<R>::lines(ny, -1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st))
       return SEP_ERR_UNSUPPORTED;
   }
+  if (ev) hipEventRecord(ev[1], st);
   // pass B (a column-major plane spectrum swaps the roles of x and y)
   const bool cm = plane && plane_is_colmajor(nx);
   if (!HOps<R>::pencil(nz, tb, N, cm ? ny : nx, cm ? nx : ny, nvec, twz, cm ? gd.kyy : gd.kxx,
                        cm ? gd.kxx : gd.kyy, gd.kzz, gd.ekin, 1.0, st))
     return SEP_ERR_UNSUPPORTED;
+  if (ev) hipEventRecord(ev[2], st);
   // pass C
   Epi<R> epi{pot, alpha, beta, part};
   if (plane) {
@@ -943,58 +947,15 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
     if (!HOps<R>::rows_bwd(nx, tb, in, out, epi, (long)ny * nz, N, nvec, twx, st, nblk))
       return SEP_ERR_UNSUPPORTED;
   }
+  if (ev) hipEventRecord(ev[3], st);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
-
-// The three passes of one application with an event recorded before each
-// pass and after the last (ev[0..3]); plane kernels only.
-template <typename R>
-int hpsi_apply_timed(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
-                     const R *pot, int nvec, double alpha, double beta, hipStream_t st,
-                     hipEvent_t *ev) {
-  const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
-  const bool plane = !gd.force_lines && plane_supported(gd.nx, gd.ny, prec);
-  const cx<R> *twx = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
-  const cx<R> *twy = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[1] : gd.tw32[1]);
-  const cx<R> *twz = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[2] : gd.tw32[2]);
-  const int nx = gd.nx, ny = gd.ny, nz = gd.nz;
-  const long N = gd.N;
-  Epi<R> epi{pot, alpha, beta, nullptr};
-  bool ok = true;
-  hipEventRecord(ev[0], st);
-  if (plane) {
-    ok &= HOps<R>::plane_fwd(nx, in, tb, N, nz, nvec, twx, st);
-  } else {
-    ok &= HOps<R>::rows_fwd(nx, in, tb, (long)ny * nz, N, nvec, twx, st);
-    ok &= HOps<R>::lines(ny, -1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st);
-  }
-  hipEventRecord(ev[1], st);
-  const bool cm = plane && plane_is_colmajor(nx);
-  ok &= HOps<R>::pencil(nz, tb, N, cm ? ny : nx, cm ? nx : ny, nvec, twz, cm ? gd.kyy : gd.kxx,
-                        cm ? gd.kxx : gd.kyy, gd.kzz, gd.ekin, 1.0, st);
-  hipEventRecord(ev[2], st);
-  if (plane) {
-    ok &= HOps<R>::plane_bwd(nx, tb, in, out, epi, N, nz, nvec, twx, st, nullptr);
-  } else {
-    ok &= HOps<R>::lines(ny, 1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st);
-    ok &= HOps<R>::rows_bwd(nx, tb, in, out, epi, (long)ny * nz, N, nvec, twx, st, nullptr);
-  }
-  hipEventRecord(ev[3], st);
-  if (!ok) return SEP_ERR_UNSUPPORTED;
-  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
-}
-template int hpsi_apply_timed<double>(const GridDev &, const VecView<double> &,
-                                      const VecView<double> &, cx<double> *, const double *, int,
-                                      double, double, hipStream_t, hipEvent_t *);
-template int hpsi_apply_timed<float>(const GridDev &, const VecView<float> &,
-                                     const VecView<float> &, cx<float> *, const float *, int,
-                                     double, double, hipStream_t, hipEvent_t *);
 
 template int hpsi_apply<double>(const GridDev &, const VecView<double> &, const VecView<double> &,
                                 cx<double> *, const double *, int, double, double, double *,
-                                long *, hipStream_t);
+                                long *, hipStream_t, hipEvent_t *);
 template int hpsi_apply<float>(const GridDev &, const VecView<float> &, const VecView<float> &,
                                cx<float> *, const float *, int, double, double, double *, long *,
-                               hipStream_t);
+                               hipStream_t, hipEvent_t *);
 
 }  // namespace sep

@@ -25,12 +25,7 @@ bool plane_supported(int nx, int ny, int prec);
 template <typename R>
 int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
                const R *pot, int nvec, double alpha, double beta, double *part, long *nblk,
-               hipStream_t st);
-
-template <typename R>
-int hpsi_apply_timed(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
-                     const R *pot, int nvec, double alpha, double beta, hipStream_t st,
-                     hipEvent_t *ev);
+               hipStream_t st, hipEvent_t *ev = nullptr);
 
 // ---- element-wise / reduction kernels (sep_kernels.hip)
 int build_potential_dev(const GridDev &gd, double xmin, double ymin, double zmin, double dx,
