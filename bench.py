@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=24)
+    p.add_argument("--no-check", action="store_true",
+                   help="skip the FP32-pipeline accuracy check (2 nm dot, 64^3)")
     p.add_argument("--kernels-only", action="store_true",
                    help="only launch the H.psi passes (FP32, 5 times) and a 256 MiB device "
                         "copy that calibrates the HBM byte counters; for rocprofv3 --pmc runs")
@@ -68,6 +70,58 @@ def measured_traffic(kernel, n, nvec):
     return None
 
 
+def precision_check(sep, decks, read_deck, np):
+    """FP32 pipeline vs FP64 pipeline on the 2.0 nm CdSe dot at 64^3 with 8
+    targets: the error of every level within 0.5 eV of the lowest converged
+    level.  16 random states and Newton length 2048 (4 and 1024 leave no
+    level of this gapless spectrum converged, sigma > 1e-3, in either
+    precision)."""
+    import tempfile
+    n, nc, ms, ns = 64, 2048, 8, 16
+    syms, xyz = decks.cdse_dot(2.0)
+    tmp = tempfile.mkdtemp(prefix="sep_check_")
+    c, h = decks.WINDOW_CENTER_HA, decks.WINDOW_HALF_HA
+    decks.write_deck(tmp, syms, xyz, n, n, n, ms, ns, nc, c - h, c + h)
+    d = read_deck(tmp)
+    ctx = sep.Context(n, n, n, d["dx"], d["dy"], d["dz"], d["xmin"], d["ymin"], d["zmin"])
+    ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
+                        expot=d["expot"])
+    vmin = ctx.vmin
+    dE, _, _ = ctx.energy_range()
+    zn, _ = sep.leja_points(nc)
+    an = sep.newton_coeffs(zn, np.linspace(d["Elmin"], d["Elmax"], ms), dE, vmin,
+                           (nc / (2.5 * dE)) ** 2)
+    N = n ** 3
+    psi0 = np.empty((ns, N))
+    idum, rng = -1001, None
+    for j in range(ns):
+        psi0[j], idum, rng = sep.init_psi(idum, N, d["dv"], rng)
+    res = {}
+    for name, prec in (("fp64", sep.SEP_PREC_FP64), ("fp32", sep.SEP_PREC_FP32)):
+        t0 = time.perf_counter()
+        ctx.filter(psi0, an, zn, vmin, dE, prec)
+        if prec == sep.SEP_PREC_FP64:
+            k, _ = ctx.ortho(ns * ms)
+            ev = ctx.rayleigh_ritz(k)
+            sg = ctx.sigma(k)
+        else:
+            k, _ = ctx.ortho_fp32(ns * ms)
+            ev, sg = ctx.refine(ctx.rayleigh_ritz(k))
+        res[name] = (ev, sg, k, time.perf_counter() - t0)
+    ctx.close()
+    ev64, sg64 = res["fp64"][0], res["fp64"][1]
+    conv = sg64 < 1e-3
+    e0 = ev64[conv].min()
+    edge = conv & (ev64 <= e0 + decks.EDGE_BAND_HA)
+    ev32 = res["fp32"][0]
+    err = [float(np.abs(ev32 - e).min() * decks.HARTREE_EV * 1e3) for e in ev64[edge]]
+    return {"config": "CdSe 2.0 nm dot, 64^3, nc 2048, 8 targets x 16 states, seed -1001",
+            "kept_fp64": res["fp64"][2], "kept_fp32": res["fp32"][2],
+            "edge_levels": int(edge.sum()), "edge_energies_Ha": [round(float(e), 8) for e in ev64[edge]],
+            "abs_err_meV": [round(e, 6) for e in err], "max_abs_err_meV": round(max(err), 6),
+            "pipeline_s": {"fp64": round(res["fp64"][3], 3), "fp32": round(res["fp32"][3], 3)}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -86,8 +140,8 @@ def main():
     t_setup = time.time()
     syms, xyz = decks.cdse_dot(args.diameter)
     tmp = tempfile.mkdtemp(prefix="sep_bench_")
-    elmid = -0.215  # mid-gap of this dot (FP64 exploratory run), +-2 eV window
-    decks.write_deck(tmp, syms, xyz, n, n, n, ms, ns, nc, elmid - 0.0735, elmid + 0.0735)
+    c, h = decks.WINDOW_CENTER_HA, decks.WINDOW_HALF_HA
+    decks.write_deck(tmp, syms, xyz, n, n, n, ms, ns, nc, c - h, c + h)
     d = read_deck(tmp)
     ctx = sep.Context(n, n, n, d["dx"], d["dy"], d["dz"], d["xmin"], d["ymin"], d["zmin"])
     ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
@@ -182,6 +236,12 @@ def main():
                 "traffic": traffic, "algorithmic_bytes": r32[dom]["bytes"],
                 "hpsi_passes_frac": r32["ABC"]["frac"]}
 
+    check = None
+    if not args.no_check:
+        check = precision_check(sep, decks, read_deck, np)
+        log(f"fp32 pipeline check: {check['edge_levels']} edge levels, max |dE| "
+            f"{check['max_abs_err_meV']:.2e} meV")
+
     # ---------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -228,6 +288,7 @@ def main():
         "roofline_passes_fp64": roof(live64, 16, 8) if live64 is not None else None,
         "accum_ms_per_block": {"fp32": round(live32[3], 4),
                                "fp64": round(live64[3], 4) if live64 is not None else None},
+        "fp32_pipeline_check": check,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

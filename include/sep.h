@@ -30,7 +30,7 @@ extern "C" {
 #define SEP_ERR_NUMERIC 6     /* factorisation failed (e.g. Cholesky) */
 
 #define SEP_PREC_FP64 0 /* complex128 filter, FP64 everywhere (parity path) */
-#define SEP_PREC_FP32 1 /* complex64 filter + FP32 MFMA subspace + FP64 refinement */
+#define SEP_PREC_FP32 1 /* complex64 filter; with sep_ortho_fp32 + sep_refine the FP32 pipeline */
 
 typedef struct sep_ctx sep_ctx;
 
@@ -189,6 +189,24 @@ int sep_rayleigh_ritz(sep_ctx *ctx, long m, double *eval);
 /* sigma_n = sqrt|<psi|H^2 psi> dv - (<psi|H psi> dv)^2| of the m resident states.
  * Replaces calc_sigma_E(), filter/energy.c:88-119. */
 int sep_sigma(sep_ctx *ctx, long m, double *sige);
+
+/* ------------------------------------------- FP32 subspace + FP64 refinement
+ * The SEP_PREC_FP32 pipeline: sep_filter (complex64) -> sep_ortho_fp32 ->
+ * sep_rayleigh_ritz (FP64) -> sep_refine (FP64).  No counterpart in the
+ * reference, which is FP64 throughout (filter/ortho.c, Hmat.c, energy.c). */
+/* Smallest singular-value ratio an FP32 Gram matrix resolves. */
+#define SEP_FP32_SVD_FLOOR 1.0e-3
+/* portho() (filter/ortho.c:11-52) with the Gram matrix formed on FP32 MFMA
+ * from an FP32 copy of the states and the basis A W S^-1 combined on FP32
+ * MFMA, then two FP64 Cholesky-QR passes.  Directions with S_i/S_0 below
+ * max(cutoff, SEP_FP32_SVD_FLOOR) are dropped. */
+int sep_ortho_fp32(sep_ctx *ctx, long m, double cutoff, double *S, long *kept);
+/* One FP64 residual-correction pass on m resident Ritz pairs with values
+ * eval: residuals r_i = H psi_i - e_i psi_i join the basis, Rayleigh-Ritz on
+ * span{psi, r} in FP64, and the m pairs of smallest sigma are kept (eval and
+ * sige overwritten, ascending in energy).  Returns SEP_ERR_NUMERIC if the
+ * enlarged basis holds fewer than m directions. */
+int sep_refine(sep_ctx *ctx, long m, double *eval, double *sige);
 
 #ifdef __cplusplus
 }

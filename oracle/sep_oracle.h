@@ -12,6 +12,14 @@
  * the reference repository root).  Arithmetic order follows the reference
  * source as written (compiled with -ffp-contract=off), so that, for the same
  * libm, results are the ones the reference source defines.
+ *
+ * The reference cannot be compiled in this image (it needs Intel icx, MKL
+ * and FFTW headers), so each function is pinned by closed forms, independent
+ * numpy/mpmath restatements and the In13_P16 fixture (tests/test_oracle_*.py).
+ * Parity unpinned: bitwise equality of or_ran_nrc with the reference binary
+ * (pinned only against an independent restatement), the Leja tie-break order
+ * under the reference binary's libm, and seed-dependent bytes of a reference
+ * run (eval-filt-*.dat, unconverged Ritz values, psi.dat).
  */
 #ifndef SEP_ORACLE_H
 #define SEP_ORACLE_H

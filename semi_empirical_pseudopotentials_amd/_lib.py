@@ -83,6 +83,8 @@ SIGNATURES = {
     "sep_ortho": (_I, [_V, _L, _D, _dp, _lp]),
     "sep_rayleigh_ritz": (_I, [_V, _L, _dp]),
     "sep_sigma": (_I, [_V, _L, _dp]),
+    "sep_ortho_fp32": (_I, [_V, _L, _D, _dp, _lp]),
+    "sep_refine": (_I, [_V, _L, _dp, _dp]),
     "sep_filter_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D, _I]),
     "sep_bench_hpsi": (_I, [_V, _I, _L, _L, _L, _dp]),
     "sep_set_timing": (_I, [_V, _I]),
@@ -391,3 +393,18 @@ class Context:
         s = np.empty(m)
         self._chk(lib().sep_sigma(self._h, m, _p(s)), "sep_sigma")
         return s
+
+    def ortho_fp32(self, m, cutoff=1.0e-10):
+        """ortho with the Gram matrix and basis on FP32 MFMA (sep_ortho_fp32)."""
+        S = np.empty(m)
+        kept = np.zeros(1, dtype=np.int64)
+        rc = lib().sep_ortho_fp32(self._h, m, cutoff, _p(S), kept.ctypes.data_as(_lp))
+        self._chk(rc, "sep_ortho_fp32")
+        return int(kept[0]), S
+
+    def refine(self, ev):
+        """One FP64 residual-correction pass on the resident Ritz pairs."""
+        ev = np.array(ev, dtype=np.float64)
+        sg = np.empty_like(ev)
+        self._chk(lib().sep_refine(self._h, ev.size, _p(ev), _p(sg)), "sep_refine")
+        return ev, sg

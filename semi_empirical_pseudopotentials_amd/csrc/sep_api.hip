@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "fft_core.h"
@@ -634,6 +635,31 @@ extern "C" int sep_filter(sep_ctx *c, const double *psi0, long ns, const double 
   return SEP_OK;
 }
 
+// Two Cholesky-QR passes in FP64 on the k resident states: G = A A^T (MFMA),
+// R = chol(G) on the host, A <- R^-T A (MFMA).  Leaves the states
+// orthonormal to FP64 rounding.
+static int cholqr2(sep_ctx *c, long k) {
+  const long N = c->gd.N;
+  RCHK(ensure_states(c, k, 2));
+  for (int pass = 0; pass < 2; pass++) {
+    const long gl = gram_work_len(k, k, N);
+    RCHK(ensure_scr(c, gl + k * k));
+    double *G_d = c->scr + gl;
+    RCHK(gram_dev<double>(c->states, k, c->states, k, N, G_d, c->scr, gl, c->st));
+    std::vector<double> G(k * k), Ri(k * k);
+    HIPCHK(c, hipMemcpyAsync(G.data(), G_d, sizeof(double) * k * k, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    int cs = sep_la_chol_upper(k, G.data());
+    if (cs) return fail(c, SEP_ERR_NUMERIC, "ortho: re-orthogonalisation Cholesky failed (%d)", cs);
+    sep_la_trinv_upper(k, G.data(), Ri.data());
+    HIPCHK(c, hipMemcpyAsync(G_d, Ri.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, c->st));
+    RCHK(combine_dev<double>(c->states, k, G_d, k, k, N, c->states2, c->st));
+    std::swap(c->states, c->states2);
+    std::swap(c->cap1, c->cap2);
+  }
+  return SEP_OK;
+}
+
 // --------------------------------------------------------------- ortho
 extern "C" int sep_ortho(sep_ctx *c, long m, double cutoff, double *S, long *kept) {
   if (!c || m < 1 || m > c->cap1 || !kept) return SEP_ERR_ARG;
@@ -688,22 +714,7 @@ extern "C" int sep_ortho(sep_ctx *c, long m, double cutoff, double *S, long *kep
   RCHK(combine_dev<double>(c->states, m, M_d, k, k, N, c->states2, c->st));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
-  for (int pass = 0; pass < 2; pass++) {
-    const long gl = gram_work_len(k, k, N);
-    RCHK(ensure_scr(c, gl + k * k));
-    double *G_d = c->scr + gl;
-    RCHK(gram_dev<double>(c->states, k, c->states, k, N, G_d, c->scr, gl, c->st));
-    std::vector<double> G(k * k), Ri(k * k);
-    HIPCHK(c, hipMemcpyAsync(G.data(), G_d, sizeof(double) * k * k, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    int cs = sep_la_chol_upper(k, G.data());
-    if (cs) return fail(c, SEP_ERR_NUMERIC, "ortho: re-orthogonalisation Cholesky failed (%d)", cs);
-    sep_la_trinv_upper(k, G.data(), Ri.data());
-    HIPCHK(c, hipMemcpyAsync(G_d, Ri.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, c->st));
-    RCHK(combine_dev<double>(c->states, k, G_d, k, k, N, c->states2, c->st));
-    std::swap(c->states, c->states2);
-    std::swap(c->cap1, c->cap2);
-  }
+  RCHK(cholqr2(c, k));
   *kept = k;
   return normalize_dev(c, c->states, k);  // filter/main.c:112
 }
@@ -734,6 +745,117 @@ extern "C" int sep_rayleigh_ritz(sep_ctx *c, long m, double *eval) {
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
   return normalize_dev(c, c->states, m);  // filter/main.c:122
+}
+
+// Orthonormal basis of the span of the m resident states from their Gram
+// matrix in R (FP32: MFMA with FP32 products; FP64: MFMA FP64): G = W L W^T
+// on the host, directions with sqrt(l_i / l_0) < floor dropped, basis
+// A W_k L_k^-1/2 formed on MFMA in R and left in c->states (FP64), then
+// polished by two FP64 Cholesky-QR passes.  S gets sqrt(l) descending.
+template <typename R>
+static int eig_basis(sep_ctx *c, long m, double floor, double *S, long *kept) {
+  const long N = c->gd.N;
+  RCHK(ensure_states(c, m, 2));
+  R *A = reinterpret_cast<R *>(c->states2);
+  R *U = A + m * N;  // FP32: the second half of states2; FP64: unused
+  if (sizeof(R) == 4) {
+    RCHK(convert_d2f(c->states, reinterpret_cast<float *>(A), m * N, c->st));
+  } else {
+    A = reinterpret_cast<R *>(c->states);
+  }
+  const long gl = gram_work_len(m, m, N);
+  RCHK(ensure_scr(c, gl + m * m));
+  double *G_d = c->scr + gl;
+  RCHK(gram_dev<R>(A, m, A, m, N, G_d, c->scr, gl, c->st));
+  std::vector<double> G(m * m), w(m);
+  HIPCHK(c, hipMemcpyAsync(G.data(), G_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (long i = 0; i < m; i++)
+    for (long j = 0; j < i; j++) G[i * m + j] = G[j * m + i] = 0.5 * (G[i * m + j] + G[j * m + i]);
+  if (sep_la_symeig(m, G.data(), w.data())) return fail(c, SEP_ERR_NUMERIC, "ortho: eig failed");
+  std::vector<double> s(m);
+  for (long i = 0; i < m; i++) s[i] = sqrt(w[m - 1 - i] > 0.0 ? w[m - 1 - i] : 0.0);
+  if (!(s[0] > 0.0)) return fail(c, SEP_ERR_NUMERIC, "ortho: all states are zero");
+  long k = m;
+  for (long i = 0; i < m; i++)
+    if (s[i] / s[0] < floor) {
+      k = i;
+      break;
+    }
+  if (S) memcpy(S, s.data(), sizeof(double) * m);
+  std::vector<double> M(m * k);
+  for (long i = 0; i < m; i++)
+    for (long j = 0; j < k; j++) M[i * k + j] = G[i * m + (m - 1 - j)] / s[j];
+  double *M_d = c->scr + gl;
+  HIPCHK(c, hipMemcpyAsync(M_d, M.data(), sizeof(double) * m * k, hipMemcpyHostToDevice, c->st));
+  if (sizeof(R) == 4) {
+    RCHK(combine_dev<float>(reinterpret_cast<float *>(A), m, M_d, k, k, N,
+                            reinterpret_cast<float *>(U), c->st));
+    RCHK(convert_f2d(reinterpret_cast<float *>(U), c->states, k * N, c->st));
+  } else {
+    RCHK(combine_dev<double>(c->states, m, M_d, k, k, N, c->states2, c->st));
+    std::swap(c->states, c->states2);
+    std::swap(c->cap1, c->cap2);
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));  // M outlives the async copy
+  RCHK(cholqr2(c, k));
+  *kept = k;
+  return normalize_dev(c, c->states, k);
+}
+
+extern "C" int sep_ortho_fp32(sep_ctx *c, long m, double cutoff, double *S, long *kept) {
+  if (!c || m < 1 || m > c->cap1 || !kept || !(cutoff >= 0.0)) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  return eig_basis<float>(c, m, cutoff > SEP_FP32_SVD_FLOOR ? cutoff : SEP_FP32_SVD_FLOOR, S, kept);
+}
+
+// One FP64 residual-correction pass on m resident Ritz pairs (values e):
+// r_i = H psi_i - e_i psi_i, normalised, Rayleigh-Ritz in span{Psi, R}
+// (orthonormalised from the FP64 Gram with singular-value floor 1e-7), and
+// the m Ritz pairs of smallest energy variance kept, ascending in energy.
+extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
+  if (!c || !eval || !sige || m < 1 || m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  // [Psi; H Psi] in states2 (2m rows), then r_i = H psi_i - e_i psi_i
+  RCHK(ensure_states(c, 2 * m, 2));
+  HIPCHK(c, hipMemcpyAsync(c->states2, c->states, sizeof(double) * m * N, hipMemcpyDeviceToDevice,
+                           c->st));
+  RCHK(apply_states(c, c->states, c->states2 + m * N, m, nullptr, nullptr));
+  RCHK(ensure_scr(c, m));
+  HIPCHK(c, hipMemcpyAsync(c->scr, eval, sizeof(double) * m, hipMemcpyHostToDevice, c->st));
+  RCHK(sub_scaled_rows_dev(c->states2 + m * N, N, c->states, N, m, N, c->scr, c->st));
+  std::swap(c->states, c->states2);
+  std::swap(c->cap1, c->cap2);
+  RCHK(normalize_dev(c, c->states + m * N, m));
+  long k = 0;
+  RCHK(eig_basis<double>(c, 2 * m, 1.0e-7, nullptr, &k));
+  std::vector<double> ev(k), sg(k);
+  RCHK(sep_rayleigh_ritz(c, k, ev.data()));
+  RCHK(sep_sigma(c, k, sg.data()));
+  // keep the m pairs of smallest variance, ascending in energy
+  std::vector<long> idx(k);
+  for (long i = 0; i < k; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](long a, long b) { return sg[a] < sg[b]; });
+  const long keep = k < m ? k : m;
+  idx.resize(keep);
+  std::sort(idx.begin(), idx.end());  // Ritz values are ascending in index
+  std::vector<double> P(k * keep, 0.0);
+  for (long j = 0; j < keep; j++) {
+    P[idx[j] * keep + j] = 1.0;
+    eval[j] = ev[idx[j]];
+    sige[j] = sg[idx[j]];
+  }
+  for (long j = keep; j < m; j++) eval[j] = sige[j] = 0.0;
+  const long gl = gram_work_len(k, keep, N);
+  RCHK(ensure_scr(c, gl + k * keep));
+  HIPCHK(c, hipMemcpyAsync(c->scr, P.data(), sizeof(double) * k * keep, hipMemcpyHostToDevice, c->st));
+  RCHK(ensure_states(c, keep, 2));
+  RCHK(combine_dev<double>(c->states, k, c->scr, keep, keep, N, c->states2, c->st));
+  std::swap(c->states, c->states2);
+  std::swap(c->cap1, c->cap2);
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return keep == m ? SEP_OK : fail(c, SEP_ERR_NUMERIC, "refine: basis of %ld < %ld states", keep, m);
 }
 
 extern "C" int sep_filter_dev(sep_ctx *c, const void *psi0_dev, long ns, const double *an,

@@ -39,6 +39,8 @@ int dots_dev(const double *a, long lda, const double *b, long ldb, long m, long 
 int dots_dev_f(const float *a, long lda, const float *b, long ldb, long m, long n, double *out_d,
                double *scratch, long scratch_len, hipStream_t st);
 // x[i*ld + g] *= s[i]
+int sub_scaled_rows_dev(double *y, long ldy, const double *x, long ldx, long m, long n,
+                        const double *s_d, hipStream_t st);
 int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStream_t st);
 int convert_d2f(const double *in, float *out, long n, hipStream_t st);
 int convert_f2d(const float *in, double *out, long n, hipStream_t st);

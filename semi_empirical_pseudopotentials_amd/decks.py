@@ -22,6 +22,20 @@ DEFAULT_POTS = {
     "P2": (-0.5, 1.0, 0.0, 1.0),
 }
 
+HARTREE_EV = 27.211386245988
+
+# Energy window of the synthetic decks.  With the tables above the dot binds
+# no state: both species' radial potentials integrate to a positive value
+# (Cd: -1.2 (pi 1.4^2)^1.5 + 0.3 (pi 3.0^2)^1.5 = +26.8 Ha bohr^3, Se: +18.5),
+# so the lowest levels (E > 0, the lowest at 0.004-0.007 Ha for 1.5-4 nm dots)
+# live in the box around the dot and the spectrum has no gap to centre the
+# +-2 eV window on.  It is centred just below the lower edge of the spectrum
+# instead, so that the targets pull out the lowest levels; "band-edge"
+# comparisons use the levels within 0.5 eV of the lowest converged one.
+WINDOW_CENTER_HA = -0.03
+WINDOW_HALF_HA = 2.0 / HARTREE_EV
+EDGE_BAND_HA = 0.5 / HARTREE_EV
+
 
 def cdse_dot(diameter_nm, lattice_A=6.08, ligand_frac=0.55, center=(0.0, 0.0, 0.0)):
     """Atoms of a zincblende CdSe sphere; returns (symbols, xyz[bohr])."""

@@ -8,7 +8,8 @@
  * Options (the reference ignores argv; these are extensions):
  *   --dir D            deck / output directory (default .)
  *   --seed S           ran_nrc seed (overrides seed.par; 0 = clock as main.c:79)
- *   --precision P      fp64 (default) or fp32 (complex64 filter)
+ *   --precision P      fp64 (default) or fp32: complex64 filter, FP32-MFMA ortho,
+ *                      FP64 Rayleigh-Ritz and one FP64 residual-correction pass
  *   --device N         HIP device ordinal
  *   --psi-filt         also write psi-filt-0-<jns>.dat (filter/filter.c:44-48)
  * Failures write the message to a file named `error` and exit 1, as the
@@ -113,7 +114,7 @@ int main(int argc, char **argv) {
   printf("Target energy for LUMO state = % .4f\n", d.Elmax);
   printf("Kinetic energy maximum = % .2f\n", 10.0);
   printf("Number of openMP threads used = %ld\n", d.nthreads);
-  printf("Precision: %s\n", prec == SEP_PREC_FP64 ? "fp64" : "fp32 filter + fp64 subspace");
+  printf("Precision: %s\n", prec == SEP_PREC_FP64 ? "fp64" : "fp32 filter/ortho + fp64 Rayleigh-Ritz and refinement");
   if (d.has_expot)
     printf("expot.par was detected: e = % .6f x0 = % .6f kT = % .6f\n", d.expot[0], d.expot[1],
            d.expot[2]);
@@ -240,7 +241,10 @@ int main(int argc, char **argv) {
   c0 = (double)clock();
   long kept = 0;
   double *S = (double *)malloc(sizeof(double) * mstot);
-  CHECK(sep_ortho(ctx, mstot, 1.0e-10, S, &kept));
+  if (prec == SEP_PREC_FP32)
+    CHECK(sep_ortho_fp32(ctx, mstot, 1.0e-10, S, &kept));
+  else
+    CHECK(sep_ortho(ctx, mstot, 1.0e-10, S, &kept));
   printf("cutoff is %ld\n", kept);
   printf("mstot ortho = %ld\n", kept);
   printf("done calculating ortho, CPU time (sec) %g, wall run time (sec) %g\n",
@@ -251,6 +255,7 @@ int main(int argc, char **argv) {
   double *eval = (double *)malloc(sizeof(double) * kept);
   double *sige = (double *)malloc(sizeof(double) * kept);
   CHECK(sep_rayleigh_ritz(ctx, kept, eval));
+  if (prec == SEP_PREC_FP32) CHECK(sep_refine(ctx, kept, eval, sige));
   printf("done calculating Hmat, CPU time (sec) %g, wall run time (sec) %g\n",
          ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
   fflush(stdout);
@@ -266,7 +271,7 @@ int main(int argc, char **argv) {
     free(psi);
   }
   /* sigma and eval.dat (main.c:135-142) */
-  CHECK(sep_sigma(ctx, kept, sige));
+  if (prec != SEP_PREC_FP32) CHECK(sep_sigma(ctx, kept, sige));  /* refine computed it */
   {
     FILE *pf = out_file("eval.dat", "w");
     if (!pf) die("cannot write eval.dat");
