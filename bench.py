@@ -236,6 +236,23 @@ def main():
                 "traffic": traffic, "algorithmic_bytes": r32[dom]["bytes"],
                 "hpsi_passes_frac": r32["ABC"]["frac"]}
 
+    # --------------------------------------- Hellmann-Feynman adjoint kernel
+    # one backward of the differentiable path: the ns*ms filtered states of the
+    # last filter pass (resident) contracted as one weighted density, and the
+    # 16 diagonal densities of the first states one by one
+    m = ns * ms
+    rng_w = np.random.default_rng(0)
+    w = rng_w.standard_normal(m)
+    diag = np.repeat(np.arange(m, dtype=np.int64)[:, None], 2, axis=1)
+    adj = {}
+    for label, pr, ww in (("weighted_density", diag, w), ("16_pairs", diag[:16], None)):
+        ctx.hf_adjoint(pr, d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"], weights=ww)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.hf_adjoint(pr, d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"], weights=ww)
+        adj[label + "_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    log(f"hf adjoint: {adj}")
+
     check = None
     if not args.no_check:
         check = precision_check(sep, decks, read_deck, np)
@@ -289,6 +306,7 @@ def main():
         "accum_ms_per_block": {"fp32": round(live32[3], 4),
                                "fp64": round(live64[3], 4) if live64 is not None else None},
         "fp32_pipeline_check": check,
+        "hf_adjoint": dict(adj, atoms=d["natom"], states=m),
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

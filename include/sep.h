@@ -190,6 +190,27 @@ int sep_rayleigh_ritz(sep_ctx *ctx, long m, double *eval);
  * Replaces calc_sigma_E(), filter/energy.c:88-119. */
 int sep_sigma(sep_ctx *ctx, long m, double *sige);
 
+/* ------------------------------------------- Hellmann-Feynman adjoint of V(r)
+ * Contractions of state densities of the resident states against the
+ * derivatives of the forward V(r) of sep_build_potential (same atom and table
+ * arguments): for each pair p = (i, j) of resident-state indices,
+ *   dR[p][a][c]    = dv sum_g psi_i psi_j dV/dR_{a,c}       (natom x 3)
+ *   dscale[p][a]   = dv sum_g psi_i psi_j dV/ds_a           (natom)
+ *   dtab[p][t][k]  = dv sum_g psi_i psi_j dV/dpot_t[k]      (ntype x stride)
+ * With weights (npairs), the pairs are first combined into the single
+ * density sum_p w_p psi_i psi_j and each output holds one slice.  With i == j
+ * these are the Hellmann-Feynman derivatives dE/d(.) of a converged state;
+ * with i != j the couplings <psi_i|dV/dR_a|psi_j> that
+ * nonAdiabaticCoupling/nonadiabatic.c:143-301 evaluates on the CPU (there
+ * with a central-difference table derivative, pseudopotential.c:6-58; here
+ * the exact derivative of the piecewise-linear interpolation,
+ * filter/interpolate.c:11-23).  Outputs are optional (NULL skips).  dtab is
+ * accumulated with atomic adds: reproducible to rounding, not bitwise. */
+int sep_hf_adjoint(sep_ctx *ctx, long npairs, const long *pairs, const double *weights,
+                   long natom, const double *xyz, const int *type, const double *scale,
+                   int ntype, const long *npot, const double *vr, const double *pot, long stride,
+                   const double *dr, double *dR, double *dscale, double *dtab);
+
 /* ------------------------------------------- FP32 subspace + FP64 refinement
  * The SEP_PREC_FP32 pipeline: sep_filter (complex64) -> sep_ortho_fp32 ->
  * sep_rayleigh_ritz (FP64) -> sep_refine (FP64).  No counterpart in the

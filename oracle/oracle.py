@@ -37,6 +37,9 @@ def lib():
         L.or_build_potential.argtypes = [_L, _L, _L, _D, _D, _D, _D, _D, _D, _L,
                                          _dp, _dp, _dp, _lp, _dp, _lp, _dp, _dp,
                                          _L, _dp, _D, _D, _D, _dp]
+        L.or_hf_adjoint.argtypes = [_L, _L, _L, _D, _D, _D, _D, _D, _D, _L, _dp, _dp, _dp,
+                                    _lp, _dp, _lp, _dp, _dp, _L, _L, _dp, _dp, _D, _dp,
+                                    _dp, _dp]
         L.or_interpolate.argtypes = [_D, _D, _dp, _dp, _L, _L, _L, _D]
         L.or_interpolate.restype = _D
         L.or_hamiltonian.argtypes = [_L, _L, _L, _dp, _dp, _dp, _L]
@@ -216,6 +219,27 @@ def build_potential(grid, origin, spacing, atoms_xyz, types, scale, npot, vr, po
                              _p(pot), vr.shape[1], _p(_c(dr)), expot[0], expot[1], expot[2],
                              _p(out))
     return out
+
+
+def hf_adjoint(grid, origin, spacing, atoms_xyz, types, scale, npot, vr, pot, dr, rho, dv):
+    """(dR natom x 3, dscale natom, dtab ntype x stride) of one density rho."""
+    nx, ny, nz = grid
+    xyz = _c(atoms_xyz)
+    rx, ry, rz = _c(xyz[:, 0]), _c(xyz[:, 1]), _c(xyz[:, 2])
+    t = _c(types, np.int64)
+    sc = _c(scale)
+    npot = _c(npot, np.int64)
+    vr = _c(vr)
+    pot = _c(pot)
+    natom = xyz.shape[0]
+    dR = np.zeros((natom, 3))
+    ds = np.zeros(natom)
+    dtab = np.zeros(vr.shape)
+    lib().or_hf_adjoint(nx, ny, nz, origin[0], origin[1], origin[2], spacing[0], spacing[1],
+                        spacing[2], natom, _p(rx), _p(ry), _p(rz), t.ctypes.data_as(_lp),
+                        _p(sc), npot.ctypes.data_as(_lp), _p(vr), _p(pot), vr.shape[1],
+                        vr.shape[0], _p(_c(dr)), _p(_c(rho)), dv, _p(dR), _p(ds), _p(dtab))
+    return dR, ds, dtab
 
 
 def run(indir, outdir, idum, nthreads=0):

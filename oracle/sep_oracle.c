@@ -347,6 +347,73 @@ int or_build_potential(long nx, long ny, long nz, double xmin, double ymin,
 }
 
 /* ========================================================================= */
+/* Hellmann-Feynman adjoint of V(r): derivatives of the forward model above  */
+/* (filter/init.c:161-182 + interpolate.c:11-23 with the strain scale)        */
+/* contracted with one density rho on the grid:                              */
+/*   dR[a]     = dv sum_g rho dV/dR_a,  dV/dR_a = -s_a slope (r - R_a)/|r-R_a| */
+/*   dscale[a] = dv sum_g rho interp(|r - R_a|)                              */
+/*   dtab[t][k]= dv sum_g rho dV/dpot_t[k]                                   */
+/* Brute force over every (grid point, atom); the product computes only the  */
+/* atoms' bounding boxes on the GPU.                                         */
+/* ========================================================================= */
+int or_hf_adjoint(long nx, long ny, long nz, double xmin, double ymin, double zmin,
+                  double dx, double dy, double dz, long natom, const double *rx,
+                  const double *ry, const double *rz, const long *type,
+                  const double *scale, const long *npot, const double *vr,
+                  const double *pot, long stride, long ntype, const double *dr,
+                  const double *rho, double dv, double *dR, double *dscale,
+                  double *dtab) {
+  double *vx = (double *)malloc(sizeof(double) * nx);
+  double *vy = (double *)malloc(sizeof(double) * ny);
+  double *vz = (double *)malloc(sizeof(double) * nz);
+  double c;
+  long j;
+  for (j = 0, c = xmin; j < nx; j++, c += dx) vx[j] = c;
+  for (j = 0, c = ymin; j < ny; j++, c += dy) vy[j] = c;
+  for (j = 0, c = zmin; j < nz; j++, c += dz) vz[j] = c;
+  memset(dtab, 0, sizeof(double) * ntype * stride);
+#pragma omp parallel for schedule(dynamic)
+  for (long a = 0; a < natom; a++) {
+    const long t = type[a];
+    const double s = scale ? scale[a] : 1.0;
+    const double *v = vr + t * stride, *p = pot + t * stride;
+    double gx = 0.0, gy = 0.0, gz = 0.0, gs = 0.0;
+    double *h = (double *)calloc(stride, sizeof(double));
+    for (long jz = 0; jz < nz; jz++)
+      for (long jy = 0; jy < ny; jy++)
+        for (long jx = 0; jx < nx; jx++) {
+          double ddx = vx[jx] - rx[a], ddy = vy[jy] - ry[a], ddz = vz[jz] - rz[a];
+          double del = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+          long i = (long)(del / dr[t]);
+          if (i > npot[t] - 2) continue;
+          double r = rho[nx * (ny * jz + jy) + jx];
+          double slope = (p[i + 1] - p[i]) / (v[i + 1] - v[i]);
+          double w = (del - v[i]) / (v[i + 1] - v[i]);
+          if (del > 0.0) {
+            double f = -r * s * slope / del;
+            gx += f * ddx;
+            gy += f * ddy;
+            gz += f * ddz;
+          }
+          gs += r * (p[i] + slope * (del - v[i]));
+          h[i] += r * s * (1.0 - w);
+          h[i + 1] += r * s * w;
+        }
+    dR[3 * a] = gx * dv;
+    dR[3 * a + 1] = gy * dv;
+    dR[3 * a + 2] = gz * dv;
+    dscale[a] = gs * dv;
+#pragma omp critical
+    for (long k = 0; k < stride; k++) dtab[t * stride + k] += h[k] * dv;
+    free(h);
+  }
+  free(vx);
+  free(vy);
+  free(vz);
+  return 0;
+}
+
+/* ========================================================================= */
 /* H application: filter/hamiltonian.c:11-44                                 */
 /* ========================================================================= */
 static void ham_one(long nx, long ny, long nz, const double *ksqr,

@@ -52,6 +52,16 @@ int or_build_potential(long nx, long ny, long nz, double xmin, double ymin,
 double or_interpolate(double r, double dr, const double *vr, const double *pot,
                       long stride, long n, long j, double scale);
 
+/* Hellmann-Feynman adjoint of or_build_potential contracted with one density
+ * rho: dR (natom x 3), dscale (natom), dtab (ntype x stride), all x dv. */
+int or_hf_adjoint(long nx, long ny, long nz, double xmin, double ymin, double zmin,
+                  double dx, double dy, double dz, long natom, const double *rx,
+                  const double *ry, const double *rz, const long *type,
+                  const double *scale, const long *npot, const double *vr,
+                  const double *pot, long stride, long ntype, const double *dr,
+                  const double *rho, double dv, double *dR, double *dscale,
+                  double *dtab);
+
 /* ---- H application (filter/hamiltonian.c:11-44) ------------------------- */
 /* phi <- T phi + V phi for `batch` complex vectors of length N. */
 int or_hamiltonian(long nx, long ny, long nz, const double *ksqr,

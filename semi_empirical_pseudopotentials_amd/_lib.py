@@ -84,6 +84,8 @@ SIGNATURES = {
     "sep_rayleigh_ritz": (_I, [_V, _L, _dp]),
     "sep_sigma": (_I, [_V, _L, _dp]),
     "sep_ortho_fp32": (_I, [_V, _L, _D, _dp, _lp]),
+    "sep_hf_adjoint": (_I, [_V, _L, _lp, _dp, _L, _dp, ctypes.POINTER(ctypes.c_int), _dp, _I,
+                            _lp, _dp, _dp, _L, _dp, _dp, _dp, _dp]),
     "sep_refine": (_I, [_V, _L, _dp, _dp]),
     "sep_filter_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D, _I]),
     "sep_bench_hpsi": (_I, [_V, _I, _L, _L, _L, _dp]),
@@ -212,6 +214,7 @@ class Context:
         self.grid = (nx, ny, nz)
         self.N = nx * ny * nz
         self.dv = dx * dy * dz
+        self.device = device
         self._desc = GridDesc(nx, ny, nz, dx, dy, dz, xmin, ymin, zmin, ekinmax, device)
         h = _V()
         rc = L.sep_open(ctypes.byref(self._desc), ctypes.byref(h))
@@ -393,6 +396,36 @@ class Context:
         s = np.empty(m)
         self._chk(lib().sep_sigma(self._h, m, _p(s)), "sep_sigma")
         return s
+
+    def hf_adjoint(self, pairs, xyz, types, npot, vr, pot, dr, scale=None, weights=None):
+        """Hellmann-Feynman contractions of resident-state densities (sep_hf_adjoint).
+
+        pairs: (P, 2) resident-state indices; weights (P) combine them into one
+        density.  Returns (dR [P', natom, 3], dscale [P', natom],
+        dtab [P', ntype, stride]) with P' = 1 if weights is given, else P."""
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int64).reshape(-1, 2))
+        npairs = pairs.shape[0]
+        xyz = _f64(xyz)
+        natom = xyz.shape[0]
+        t = np.ascontiguousarray(np.asarray(types, dtype=np.int32))
+        vr = _f64(vr)
+        pot = _f64(pot)
+        ntype, stride = vr.shape
+        npot = np.ascontiguousarray(np.asarray(npot, dtype=np.int64))
+        nout = 1 if weights is not None else npairs
+        dR = np.zeros((nout, natom, 3))
+        ds = np.zeros((nout, natom))
+        dtab = np.zeros((nout, ntype, stride))
+        w = _f64(weights) if weights is not None else None
+        sc = _f64(scale) if scale is not None else None
+        rc = lib().sep_hf_adjoint(self._h, npairs, pairs.ctypes.data_as(_lp),
+                                  _p(w) if w is not None else None, natom, _p(xyz),
+                                  t.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                  _p(sc) if sc is not None else None, ntype,
+                                  npot.ctypes.data_as(_lp), _p(vr), _p(pot), stride, _p(_f64(dr)),
+                                  _p(dR), _p(ds), _p(dtab))
+        self._chk(rc, "sep_hf_adjoint")
+        return dR, ds, dtab
 
     def ortho_fp32(self, m, cutoff=1.0e-10):
         """ortho with the Gram matrix and basis on FP32 MFMA (sep_ortho_fp32)."""

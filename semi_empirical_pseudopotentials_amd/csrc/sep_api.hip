@@ -95,6 +95,30 @@ static int ensure_scr(sep_ctx *c, size_t len) {
   return SEP_OK;
 }
 
+// Temporary device allocation freed on scope exit.
+struct DevBuf {
+  void *p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+static int dev_alloc(sep_ctx *c, DevBuf &b, size_t bytes) {
+  HIPCHK(c, hipMalloc(&b.p, bytes > 0 ? bytes : 8));
+  return SEP_OK;
+}
+static int dev_upload(sep_ctx *c, DevBuf &b, const void *src, size_t bytes) {
+  RCHK(dev_alloc(c, b, bytes));
+  HIPCHK(c, hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+  return SEP_OK;
+}
+
 extern "C" {
 
 const char *sep_last_error(const sep_ctx *c) { return c ? c->err.c_str() : "null context"; }
@@ -243,40 +267,25 @@ int sep_build_potential(sep_ctx *c, long natom, const double *xyz, const int *ty
   const long N = c->gd.N;
   size_t cap = 0;
   if (!c->pot64) RCHK(ensure(c, (void **)&c->pot64, &cap, sizeof(double) * N));
-  double *xyz_d, *scale_d = nullptr, *vr_d, *pot_d, *dr_d;
-  int *type_d;
-  long *npot_d;
-  HIPCHK(c, hipMalloc(&xyz_d, sizeof(double) * 3 * natom));
-  HIPCHK(c, hipMalloc(&type_d, sizeof(int) * natom));
-  HIPCHK(c, hipMalloc(&npot_d, sizeof(long) * ntype));
-  HIPCHK(c, hipMalloc(&vr_d, sizeof(double) * ntype * stride));
-  HIPCHK(c, hipMalloc(&pot_d, sizeof(double) * ntype * stride));
-  HIPCHK(c, hipMalloc(&dr_d, sizeof(double) * ntype));
-  if (scale) HIPCHK(c, hipMalloc(&scale_d, sizeof(double) * natom));
-  HIPCHK(c, hipMemcpy(xyz_d, xyz, sizeof(double) * 3 * natom, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(type_d, type, sizeof(int) * natom, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(npot_d, npot, sizeof(long) * ntype, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(vr_d, vr, sizeof(double) * ntype * stride, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(pot_d, pot, sizeof(double) * ntype * stride, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(dr_d, dr, sizeof(double) * ntype, hipMemcpyHostToDevice));
-  if (scale) HIPCHK(c, hipMemcpy(scale_d, scale, sizeof(double) * natom, hipMemcpyHostToDevice));
+  DevBuf xyz_d, type_d, npot_d, vr_d, pot_d, dr_d, scale_d;
+  RCHK(dev_upload(c, xyz_d, xyz, sizeof(double) * 3 * natom));
+  RCHK(dev_upload(c, type_d, type, sizeof(int) * natom));
+  RCHK(dev_upload(c, npot_d, npot, sizeof(long) * ntype));
+  RCHK(dev_upload(c, vr_d, vr, sizeof(double) * ntype * stride));
+  RCHK(dev_upload(c, pot_d, pot, sizeof(double) * ntype * stride));
+  RCHK(dev_upload(c, dr_d, dr, sizeof(double) * ntype));
+  if (scale) RCHK(dev_upload(c, scale_d, scale, sizeof(double) * natom));
   double ex_e = 0.0, ex_x0 = 0.0, ex_t = 1.0;
   if (expot) {
     ex_e = expot[0];
     ex_x0 = expot[1];
     ex_t = expot[2];
   }
-  int rc = build_potential_dev(c->gd, c->xmin, c->ymin, c->zmin, c->dx, c->dy, c->dz, natom, xyz_d,
-                               type_d, scale_d, npot_d, vr_d, pot_d, stride, dr_d, ex_e, ex_x0,
-                               ex_t, c->pot64, c->st);
-  hipStreamSynchronize(c->st);
-  hipFree(xyz_d);
-  hipFree(type_d);
-  hipFree(npot_d);
-  hipFree(vr_d);
-  hipFree(pot_d);
-  hipFree(dr_d);
-  if (scale_d) hipFree(scale_d);
+  int rc = build_potential_dev(c->gd, c->xmin, c->ymin, c->zmin, c->dx, c->dy, c->dz, natom,
+                               xyz_d.as<double>(), type_d.as<int>(), scale_d.as<double>(),
+                               npot_d.as<long>(), vr_d.as<double>(), pot_d.as<double>(), stride,
+                               dr_d.as<double>(), ex_e, ex_x0, ex_t, c->pot64, c->st);
+  HIPCHK(c, hipStreamSynchronize(c->st));
   if (rc != SEP_OK) return fail(c, rc, "potential kernel launch failed");
   RCHK(finish_potential(c));
   if (potl_out)
@@ -856,6 +865,77 @@ extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
   std::swap(c->cap1, c->cap2);
   HIPCHK(c, hipStreamSynchronize(c->st));
   return keep == m ? SEP_OK : fail(c, SEP_ERR_NUMERIC, "refine: basis of %ld < %ld states", keep, m);
+}
+
+// ------------------------------------------------- Hellmann-Feynman adjoint
+extern "C" int sep_hf_adjoint(sep_ctx *c, long npairs, const long *pairs, const double *weights,
+                              long natom, const double *xyz, const int *type, const double *scale,
+                              int ntype, const long *npot, const double *vr, const double *pot,
+                              long stride, const double *dr, double *dR, double *dscale,
+                              double *dtab) {
+  if (!c || npairs < 1 || !pairs || natom < 1 || !xyz || !type || ntype < 1 || !npot || !vr ||
+      !pot || !dr || stride < 2)
+    return SEP_ERR_ARG;
+  for (long p = 0; p < 2 * npairs; p++)
+    if (pairs[p] < 0 || pairs[p] >= c->cap1)
+      return fail(c, SEP_ERR_ARG, "pair %ld: state %ld not resident", p / 2, pairs[p]);
+  long maxnpot = 0;
+  for (long a = 0; a < natom; a++) {
+    if (type[a] < 0 || type[a] >= ntype || npot[type[a]] < 2 || npot[type[a]] > stride ||
+        !(dr[type[a]] > 0.0))
+      return fail(c, SEP_ERR_ARG, "atom %ld: bad type or table", a);
+    if (npot[type[a]] > maxnpot) maxnpot = npot[type[a]];
+  }
+  if (hf_pairs_per_pass(maxnpot) == 0)
+    return fail(c, SEP_ERR_ARG, "tables of %ld rows exceed the LDS histogram", maxnpot);
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  const long np = weights ? 1 : npairs;
+  // box points per atom -> slices per atom (>= ~4096 points per block)
+  double rmax = 0.0;
+  for (int t = 0; t < ntype; t++)
+    if (npot[t] >= 2 && (double)(npot[t] - 1) * dr[t] > rmax) rmax = (double)(npot[t] - 1) * dr[t];
+  const double box = (2.0 * rmax / c->dx + 1.0) * (2.0 * rmax / c->dy + 1.0) * (2.0 * rmax / c->dz + 1.0);
+  long nsplit = (long)(box / 4096.0) + 1;
+  if (nsplit > 64) nsplit = 64;
+  const int pb = hf_pairs_per_pass(maxnpot);
+  DevBuf xyz_d, type_d, npot_d, vr_d, pot_d, dr_d, scale_d, pairs_d, w_d, rho_d, work_d, htab_d,
+      dR_d, ds_d;
+  RCHK(dev_upload(c, xyz_d, xyz, sizeof(double) * 3 * natom));
+  RCHK(dev_upload(c, type_d, type, sizeof(int) * natom));
+  RCHK(dev_upload(c, npot_d, npot, sizeof(long) * ntype));
+  RCHK(dev_upload(c, vr_d, vr, sizeof(double) * ntype * stride));
+  RCHK(dev_upload(c, pot_d, pot, sizeof(double) * ntype * stride));
+  RCHK(dev_upload(c, dr_d, dr, sizeof(double) * ntype));
+  if (scale) RCHK(dev_upload(c, scale_d, scale, sizeof(double) * natom));
+  RCHK(dev_upload(c, pairs_d, pairs, sizeof(long) * 2 * npairs));
+  if (weights) RCHK(dev_upload(c, w_d, weights, sizeof(double) * npairs));
+  RCHK(dev_alloc(c, rho_d, sizeof(double) * np * N));
+  RCHK(dev_alloc(c, work_d, sizeof(double) * natom * nsplit * pb * 4));
+  RCHK(dev_alloc(c, htab_d, sizeof(double) * np * ntype * stride));
+  RCHK(dev_alloc(c, dR_d, sizeof(double) * np * natom * 3));
+  RCHK(dev_alloc(c, ds_d, sizeof(double) * np * natom));
+  HIPCHK(c, hipMemsetAsync(htab_d.p, 0, sizeof(double) * np * ntype * stride, c->st));
+  RCHK(pair_density_dev(c->states, N, pairs_d.as<long>(), w_d.as<double>(), (int)npairs,
+                        rho_d.as<double>(), c->st));
+  int rc = hf_adjoint_dev(c->gd, c->xmin, c->ymin, c->zmin, c->dx, c->dy, c->dz, c->dv, (int)natom,
+                          xyz_d.as<double>(), type_d.as<int>(), scale_d.as<double>(),
+                          npot_d.as<long>(), maxnpot, vr_d.as<double>(), pot_d.as<double>(), stride,
+                          dr_d.as<double>(), ntype, rho_d.as<double>(), (int)np, (int)nsplit,
+                          work_d.as<double>(), htab_d.as<double>(), dR_d.as<double>(),
+                          ds_d.as<double>(), c->st);
+  if (rc != SEP_OK) return fail(c, rc, "adjoint kernel launch failed");
+  if (dR)
+    HIPCHK(c, hipMemcpyAsync(dR, dR_d.p, sizeof(double) * np * natom * 3, hipMemcpyDeviceToHost, c->st));
+  if (dscale)
+    HIPCHK(c, hipMemcpyAsync(dscale, ds_d.p, sizeof(double) * np * natom, hipMemcpyDeviceToHost, c->st));
+  if (dtab)
+    HIPCHK(c, hipMemcpyAsync(dtab, htab_d.p, sizeof(double) * np * ntype * stride,
+                             hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (dtab)
+    for (long i = 0; i < np * ntype * stride; i++) dtab[i] *= c->dv;
+  return SEP_OK;
 }
 
 extern "C" int sep_filter_dev(sep_ctx *c, const void *psi0_dev, long ns, const double *an,

@@ -39,6 +39,17 @@ int dots_dev(const double *a, long lda, const double *b, long ldb, long m, long 
 int dots_dev_f(const float *a, long lda, const float *b, long ldb, long m, long n, double *out_d,
                double *scratch, long scratch_len, hipStream_t st);
 // x[i*ld + g] *= s[i]
+// ---- Hellmann-Feynman adjoint (sep_adjoint.hip)
+int pair_density_dev(const double *states, long N, const long *pairs_d, const double *w_d, int np,
+                     double *rho, hipStream_t st);
+int hf_pairs_per_pass(long maxnpot);
+int hf_adjoint_dev(const GridDev &gd, double xmin, double ymin, double zmin, double dx, double dy,
+                   double dz, double dv, int natom, const double *xyz, const int *type,
+                   const double *scale, const long *npot, long maxnpot, const double *vr,
+                   const double *pot, long stride, const double *dr, int ntype, const double *rho,
+                   int np, int nsplit, double *work, double *htab, double *dR, double *dscale,
+                   hipStream_t st);
+
 int sub_scaled_rows_dev(double *y, long ldy, const double *x, long ldx, long m, long n,
                         const double *s_d, hipStream_t st);
 int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStream_t st);
