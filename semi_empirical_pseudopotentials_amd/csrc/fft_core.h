@@ -26,8 +26,10 @@
 
 namespace sep {
 
+// Aligned to its full size so that LDS and global accesses of one element
+// are single 8-byte (FP32) / 16-byte (FP64) operations.
 template <typename R>
-struct cx {
+struct alignas(2 * sizeof(R)) cx {
   R re, im;
 };
 

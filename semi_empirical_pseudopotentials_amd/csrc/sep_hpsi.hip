@@ -534,7 +534,9 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
   constexpr int rl_ = Pass<NX, 1>::radix(P - 1);
   constexpr int NVR = NH == 1 ? 1 : NR;  // row inputs held in registers
   cx<R> v[NVR][E];
-  TwFor<NX, 1, R> tc;
+  // at 1024 threads the register budget is 128: twiddles come from LDS
+  using TwB = typename std::conditional<(T >= 1024), TwTable<NX, 1, R>, TwFor<NX, 1, R>>::type;
+  TwB tc;
 #pragma unroll
   for (int c = 0; c < NH; c++) {
     // ---- columns of chunk c: load, y FFT, into the buffer
@@ -592,9 +594,13 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
     const int row = r * RX + tl;
     const bool act = row < NX;
     const int rs = act ? row : 0;
+    // psi_{j-1} and V of the row: loaded before the FFT (in flight during
+    // it) at 512 threads; after it at 1024, where the register budget is 128
+    // and the other waves hide the latency
+    constexpr bool early = T < 1024;
     cx<R> p[E];
     R V[E];
-    if (act) {
+    if (early && act) {
 #pragma unroll
       for (int b = 0; b < E / rl_; b++)
 #pragma unroll
@@ -617,6 +623,16 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
       run_passes_c<NX, 1, 0, P, WaveSync>(x, g, act, tc, at);
     }
     if (act) {
+      if constexpr (!early) {
+#pragma unroll
+        for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+          for (int k = 0; k < rl_; k++) {
+            const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
+            p[b * rl_ + k] = pin.ld(vec, gi);
+            V[b * rl_ + k] = epi.pot[gi];
+          }
+      }
 #pragma unroll
       for (int b = 0; b < E / rl_; b++)
 #pragma unroll
@@ -646,7 +662,10 @@ constexpr int plane_threads() {
     // lines per round limited so that the row-exchange slices fit the chunk buffer
     constexpr int G = FftPlan<NX>::G;
     constexpr int LDc = NX / plane_chunks<R, NX>() + 1;
-    int t = NX >= 64 ? 512 : 256;
+    // whole-plane buffers take one workgroup per CU: 1024 threads (16 waves)
+    // hide more latency and hold half a plane's points per thread of 512
+    int t = sizeof(R) == 4 && NX >= 96 && plane_chunks<R, NX>() == 1 ? 1024
+                                                                      : (NX >= 64 ? 512 : 256);
     if (plane_chunks<R, NX>() > 1)
       while (t > 64 && (t / G) * (NX + 1) > NX * LDc) t -= 64;
     return t;
