@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(T) k_plane_bwd(const cx<R> *tb, VecView<R> pin
 // When the threads of one line fit a wavefront (64 % G == 0), every FFT
 // exchange of a row or a column stays inside one wavefront; only the
 // row <-> column transpose needs block barriers.  Line positions in LDS are
-// xor-swizzled (lds_swz) so the strided exchange patterns spread over the
+// placed by PlaneLds (below) so the strided exchange patterns spread over the
 // banks; FP32 threads keep their twiddles in registers (TwFor).
 // If the plane does not fit LDS, the transpose goes through the buffer in
 // NH chunks of NX/NH columns while the plane stays in registers.
@@ -425,20 +425,52 @@ constexpr int plane_chunks() {
   return nh;
 }
 
+// LDS placement of one plane in the wave-local plane kernels: row y of the
+// (transposed) plane starts at rowbase(y); during the row FFTs element x of
+// a row sits at rowbase(y) + xpos(x).  Generic lengths: pitch NX/NH + 1,
+// rows and in-row positions xor-swizzled.  Complex64 at 128 (G = 16 threads
+// per line, two lines per half-wave): pitch 150 with per-row offsets and an
+// in-row permutation that make every access of the row and column phases
+// free of bank conflicts for 8-byte accesses on gfx950 (64 banks of 4 B, 32
+// lanes served per cycle): the two rows (columns) of a half-wave start
+// 16 mod 32 elements apart, and the 16 positions of every access pattern of
+// the radix-8/8/2 passes are distinct mod 16.
+template <typename R, int NX, int NH>
+struct PlaneLds {
+  static constexpr int LDc = NX / NH + 1;
+  static __device__ __forceinline__ int rowbase(int y) { return lds_swz(y) * LDc; }
+  static __device__ __forceinline__ int xpos(int x) { return lds_swz(x); }
+};
+template <>
+struct PlaneLds<float, 128, 1> {
+  static constexpr int LDc = 150;
+  static __device__ __forceinline__ int rowbase(int y) {
+    const int d = 3 + 2 * ((y >> 5) & 3) + ((((y >> 3) ^ (y >> 4)) & 1) ^ ((y >> 6) & 1)) -
+                  3 * (y & 1);
+    return y * LDc + 2 * d;
+  }
+  static __device__ __forceinline__ int xpos(int x) {
+    const int u = x >> 4;
+    return x ^ (((9 * u) ^ (8 * (u >> 2))) & 15);
+  }
+};
+
 template <typename R, int NX, int T>
 __global__ void __launch_bounds__(T) k_plane_fwd_w(VecView<R> in, cx<R> *tb, long N, int nz,
                                                    const cx<R> *twg) {
   using PL = FftPlan<NX>;
   constexpr int G = PL::G, E = PL::E, P = PL::P;
   static_assert(64 % G == 0 && NX % 8 == 0, "lines must not straddle wavefronts");
-  constexpr int NH = plane_chunks<R, NX>(), W = NX / NH, LDc = W + 1;
+  constexpr int NH = plane_chunks<R, NX>(), W = NX / NH;
+  using L = PlaneLds<R, NX, NH>;
+  constexpr int LDc = L::LDc;
   constexpr int RX = T / G;                  // lines per round
   constexpr int NR = (NX + RX - 1) / RX;     // row rounds
   constexpr int NCR = (W + RX - 1) / RX;     // column rounds per chunk
   constexpr int RS = NX + 1;                 // row-exchange slice pitch (NH > 1)
   static_assert(NH == 1 || RX * RS <= NX * LDc, "row exchange slices exceed the chunk buffer");
   extern __shared__ char smem_raw[];
-  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);  // [NX][LDc], row y at lds_swz(y)
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);  // [NX][LDc], row y at L::rowbase(y)
   cx<R> *tw = buf + NX * LDc;
   const long vec = blockIdx.x / nz;
   const long z = blockIdx.x % nz;
@@ -468,8 +500,8 @@ __global__ void __launch_bounds__(T) k_plane_fwd_w(VecView<R> in, cx<R> *tb, lon
   for (int r = 0; r < NR; r++) {
     const int row = r * RX + tl;
     const bool act = row < NX;
-    const int base = (NH == 1 ? (act ? row : 0) * LDc : tl * RS);
-    auto at = [&](int i) -> cx<R> & { return buf[base + lds_swz(i)]; };
+    const int base = (NH == 1 ? L::rowbase(act ? row : 0) : tl * RS);
+    auto at = [&](int i) -> cx<R> & { return buf[base + L::xpos(i)]; };
     run_passes_c<NX, -1, 0, P, WaveSync>(v[r], g, act, tc, at);
   }
   // ---- columns, chunk by chunk
@@ -485,7 +517,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_w(VecView<R> in, cx<R> *tb, lon
 #pragma unroll
           for (int k = 0; k < rl_; k++) {
             const int kx = out_idx<NX, -1, P - 1>(g, b, k);
-            if (kx / W == c) buf[lds_swz(row) * LDc + (kx - c * W)] = v[r][b * rl_ + k];
+            if (kx / W == c) buf[L::rowbase(row) + (kx - c * W)] = v[r][b * rl_ + k];
           }
       }
     }
@@ -496,7 +528,7 @@ __global__ void __launch_bounds__(T) k_plane_fwd_w(VecView<R> in, cx<R> *tb, lon
       const bool act = cl < W;
       const int cs = act ? cl : 0;
       cx<R> w[E];
-      auto at = [&](int i) -> cx<R> & { return buf[lds_swz(i) * LDc + cs]; };
+      auto at = [&](int i) -> cx<R> & { return buf[L::rowbase(i) + cs]; };
       read_in<NX, -1, 0>(w, g, act, at);
       run_passes_c<NX, -1, 0, P, WaveSync>(w, g, act, tc, at);
       if (act) {
@@ -517,7 +549,9 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
   using PL = FftPlan<NX>;
   constexpr int G = PL::G, E = PL::E, P = PL::P;
   static_assert(64 % G == 0 && NX % 8 == 0, "lines must not straddle wavefronts");
-  constexpr int NH = plane_chunks<R, NX>(), W = NX / NH, LDc = W + 1;
+  constexpr int NH = plane_chunks<R, NX>(), W = NX / NH;
+  using L = PlaneLds<R, NX, NH>;
+  constexpr int LDc = L::LDc;
   constexpr int RX = T / G;
   constexpr int NR = (NX + RX - 1) / RX;
   constexpr int NCR = (W + RX - 1) / RX;
@@ -564,7 +598,7 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
       const int cl = cr * RX + tl;
       const bool act = cl < W;
       const int cs = act ? cl : 0;
-      auto at = [&](int i) -> cx<R> & { return buf[lds_swz(i) * LDc + cs]; };
+      auto at = [&](int i) -> cx<R> & { return buf[L::rowbase(i) + cs]; };
       run_passes_c<NX, 1, 0, P, WaveSync>(w[cr], g, act, tc, at);
       WaveSync::sync();
       write_out<NX, 1, P - 1>(w[cr], g, act, at);
@@ -581,7 +615,7 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
 #pragma unroll
             for (int k = 0; k < r0; k++) {
               const int x = in_idx<NX, 1, 0>(g, b, k);
-              if (x / W == c) v[r][b * r0 + k] = buf[lds_swz(row) * LDc + (x - c * W)];
+              if (x / W == c) v[r][b * r0 + k] = buf[L::rowbase(row) + (x - c * W)];
             }
         }
       }
@@ -612,11 +646,11 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
     }
     cx<R> *x = v[NH == 1 ? 0 : r];
     if constexpr (NH == 1) {
-      // the row is in the buffer (row y at lds_swz(y), x unswizzled); the
-      // exchanges then run in place in that row with swizzled x
-      auto at0 = [&](int i) -> cx<R> & { return buf[lds_swz(rs) * LDc + i]; };
+      // the row is in the buffer (at L::rowbase(y), x in natural order); the
+      // exchanges then run in place in that row at the permuted positions
+      auto at0 = [&](int i) -> cx<R> & { return buf[L::rowbase(rs) + i]; };
       read_in<NX, 1, 0>(x, g, act, at0);
-      auto at = [&](int i) -> cx<R> & { return buf[lds_swz(rs) * LDc + lds_swz(i)]; };
+      auto at = [&](int i) -> cx<R> & { return buf[L::rowbase(rs) + L::xpos(i)]; };
       run_passes_c<NX, 1, 0, P, WaveSync>(x, g, act, tc, at);
     } else {
       auto at = [&](int i) -> cx<R> & { return buf[tl * RS + lds_swz(i)]; };
@@ -653,7 +687,7 @@ constexpr bool plane_colmajor() {
 template <typename R, int NX>
 constexpr size_t plane_lds() {
   if constexpr (plane_colmajor<NX>())
-    return sizeof(cx<R>) * ((size_t)NX * (NX / plane_chunks<R, NX>() + 1) + NX);
+    return sizeof(cx<R>) * ((size_t)NX * PlaneLds<R, NX, plane_chunks<R, NX>()>::LDc + NX);
   return sizeof(cx<R>) * ((size_t)NX * PlaneCfg<NX>::LD + NX);
 }
 template <typename R, int NX>
@@ -782,6 +816,7 @@ static void plane_fwd_launch(VecView<R> in, cx<R> *tb, long N, int nz, int nvec,
                              hipStream_t st) {
   constexpr int T = plane_threads<R, n>();
   constexpr size_t lds = plane_lds<R, n>();
+  static_assert(lds <= 160 * 1024, "plane buffer exceeds the LDS of a gfx950 workgroup");
   static bool attr = false;
   if constexpr (plane_colmajor<n>()) {
     if (!attr) {
@@ -806,6 +841,7 @@ static void plane_bwd_launch(const cx<R> *tb, VecView<R> pin, VecView<R> out, Ep
                              int nz, int nvec, const cx<R> *tw, hipStream_t st) {
   constexpr int T = plane_threads<R, n>();
   constexpr size_t lds = plane_lds<R, n>();
+  static_assert(lds <= 160 * 1024, "plane buffer exceeds the LDS of a gfx950 workgroup");
   static bool attr = false;
   if constexpr (plane_colmajor<n>()) {
     if (!attr) {
