@@ -11,6 +11,8 @@
 // (measured on gfx950).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sep_internal.h"
 
 namespace sep {
@@ -270,12 +272,100 @@ __global__ void __launch_bounds__(256) k_accum_mfma(const R *__restrict__ ring, 
       }
 }
 
+// The same product with 16-byte loads: lane li of a wave loads VW = 16 /
+// sizeof(R) consecutive points of one ring row per instruction, and MFMA
+// column j = li of y-tile y stands for point g0 + (y / VW) 16 VW + VW li +
+// y % VW, so that a wave's 64 points of one row come in 4 / VW loads of
+// 16 x 16 bytes and its accumulator rows are updated in VW-point runs.
+// Needs N % 256 == 0 (every block full).
+template <typename R>
+__global__ void __launch_bounds__(256) k_accum_mfma_v(const R *__restrict__ ring, long slot,
+                                                      int cnt, const double *__restrict__ anT,
+                                                      long j0, int ms, double *__restrict__ acc,
+                                                      long N) {
+  constexpr int VW = 16 / sizeof(R);
+  constexpr int NH = 4 / VW;  // 16-byte loads per row and k-step
+  using V = typename std::conditional<sizeof(R) == 4, float4, double2>::type;
+  const long s = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long g0 = (long)blockIdx.x * 256 + w * 64;
+  const int ie0 = blockIdx.z * 32;
+  const int kq = lane >> 4, li = lane & 15;
+  const R *rp = ring + s * N + g0 + VW * li;
+  typename Mfma<R>::acc_t d[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) d[x][y] = typename Mfma<R>::acc_t{0, 0, 0, 0};
+  // one k-step: 4 ring rows (kq) x 64 points, 2 x 4 MFMA tiles
+  auto load = [&](int kk, bool kin, R *a, R *b) {
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      const int ie = ie0 + 16 * x + li;
+      a[x] = (kin && ie < ms) ? (R)anT[(j0 + kk) * ms + ie] : (R)0;
+    }
+#pragma unroll
+    for (int h = 0; h < NH; h++) {
+      V v{};
+      if (kin) v = *reinterpret_cast<const V *>(rp + (long)kk * slot + h * 16 * VW);
+      const R *vp = reinterpret_cast<const R *>(&v);
+#pragma unroll
+      for (int q = 0; q < VW; q++) b[h * VW + q] = vp[q];
+    }
+  };
+  auto mma = [&](const R *a, const R *b) {
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) d[x][y] = Mfma<R>::op(a[x], b[y], d[x][y]);
+  };
+  // U k-steps per iteration with all their loads issued first (more bytes in
+  // flight per wave), then the remainder one step at a time
+  constexpr int U = 4;
+  int k = 0;
+  for (; k + 4 * U <= cnt; k += 4 * U) {
+    R a[U][2], b[U][4];
+#pragma unroll
+    for (int u = 0; u < U; u++) load(k + 4 * u + kq, true, a[u], b[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++) mma(a[u], b[u]);
+  }
+  for (; k < cnt; k += 4) {
+    R a[2], b[4];
+    load(k + kq, k + kq < cnt, a, b);
+    mma(a, b);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int ie = ie0 + 16 * x + Mfma<R>::row(lane, r);
+      if (ie >= ms) continue;
+      double *ap = acc + (s * ms + ie) * N + g0 + VW * li;
+#pragma unroll
+      for (int h = 0; h < NH; h++) {
+        double2 *p2 = reinterpret_cast<double2 *>(ap + h * 16 * VW);
+#pragma unroll
+        for (int q = 0; q < VW / 2; q++) {
+          double2 o = p2[q];
+          o.x += (double)d[x][h * VW + 2 * q][r];
+          o.y += (double)d[x][h * VW + 2 * q + 1][r];
+          p2[q] = o;
+        }
+      }
+    }
+}
+
 template <typename R>
 int accum_mfma_dev(const R *ring, long slot, int cnt, const double *anT_d, long j0, int ms,
                    double *acc, int ns, long N, hipStream_t st) {
   dim3 grid((unsigned)((N + 255) / 256), (unsigned)ns, (unsigned)((ms + 31) / 32));
-  hipLaunchKernelGGL(k_accum_mfma<R>, grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms, acc,
-                     N);
+  if (N % 256 == 0 && slot % 4 == 0)
+    hipLaunchKernelGGL(k_accum_mfma_v<R>, grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms,
+                       acc, N);
+  else
+    hipLaunchKernelGGL(k_accum_mfma<R>, grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms,
+                       acc, N);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 template int accum_mfma_dev<double>(const double *, long, int, const double *, long, int, double *,
