@@ -144,6 +144,23 @@ int sep_gram_dev(sep_ctx *ctx, int precision, const void *A_dev, long m, const v
 int sep_rotate_dev(sep_ctx *ctx, int precision, const void *A_dev, long m, const double *M,
                    long mo, void *Out_dev);
 
+/* ------------------------------------------------- batched uneven grids
+ * sep_filter_dev for ndots nanocrystals at once, each with its own context
+ * (grid, potential, stream) and its own random states, coefficients and
+ * sizes; all contexts on one device.  mode SEP_BATCH_INTERLEAVE enqueues one
+ * Newton step of every unfinished grid per round on the grids' streams, so
+ * that kernels of small grids run beside those of large ones;
+ * SEP_BATCH_GRAPH captures each grid's whole step sequence into one HIP
+ * graph and launches the graphs on the grids' streams.  Results as
+ * sep_filter_dev, per context.  No counterpart in the reference, which runs
+ * one nanocrystal per process (filter/main.c). */
+#define SEP_BATCH_INTERLEAVE 0
+#define SEP_BATCH_GRAPH 1
+int sep_filter_batch(int ndots, sep_ctx *const *ctxs, const void *const *psi0_dev, const long *ns,
+                     const double *const *an, const double *const *zn, const long *nc,
+                     const long *ms, const double *vmin, const double *dE, int precision,
+                     int mode);
+
 /* ------------------------------------------------------------ measurement */
 /* Times the H.psi passes on nvec resident complex vectors (random data) with
  * HIP events on the context's stream: kernel_ms[0..3] = average duration of

@@ -90,6 +90,7 @@ SIGNATURES = {
     "sep_filter_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D, _I]),
     "sep_bench_hpsi": (_I, [_V, _I, _L, _L, _L, _dp]),
     "sep_set_timing": (_I, [_V, _I]),
+    "sep_filter_batch": (_I, [_I, _V, _V, _lp, _V, _V, _lp, _lp, _dp, _dp, _I, _I]),
     "sep_get_timing": (_I, [_V, _dp, ctypes.POINTER(ctypes.c_long)]),
     "sep_gram_dev": (_I, [_V, _I, _V, _L, _V, _L, _dp]),
     "sep_rotate_dev": (_I, [_V, _I, _V, _L, _dp, _L, _V]),
@@ -203,6 +204,37 @@ def read_deck(path):
 
 def fft_length_supported(n):
     return bool(lib().sep_fft_length_supported(n))
+
+
+SEP_BATCH_INTERLEAVE = 0
+SEP_BATCH_GRAPH = 1
+
+
+def filter_batch(ctxs, psi0_ptrs, ns, ans, zns, vmins, dEs, precision=SEP_PREC_FP32,
+                 mode=SEP_BATCH_INTERLEAVE):
+    """sep_filter_batch: one filter pass on several contexts (grids) at once.
+
+    psi0_ptrs: device pointers of each grid's ns x N FP64 random states;
+    ans: (ms, nc) coefficient arrays; zns: Leja points; per-grid vmin and dE."""
+    nd = len(ctxs)
+    ans = [_f64(a) for a in ans]
+    zns = [_f64(z) for z in zns]
+    H = (_V * nd)(*[c._h for c in ctxs])
+    P = (_V * nd)(*[_V(p) for p in psi0_ptrs])
+    A = (_dp * nd)(*[_p(a) for a in ans])
+    Z = (_dp * nd)(*[_p(z) for z in zns])
+    nsv = np.ascontiguousarray(ns, dtype=np.int64)
+    ncv = np.ascontiguousarray([a.shape[1] for a in ans], dtype=np.int64)
+    msv = np.ascontiguousarray([a.shape[0] for a in ans], dtype=np.int64)
+    vm = _f64(vmins)
+    de = _f64(dEs)
+    rc = lib().sep_filter_batch(nd, ctypes.cast(H, _V), ctypes.cast(P, _V),
+                                nsv.ctypes.data_as(_lp), ctypes.cast(A, _V), ctypes.cast(Z, _V),
+                                ncv.ctypes.data_as(_lp), msv.ctypes.data_as(_lp), _p(vm), _p(de),
+                                precision, mode)
+    if rc != SEP_OK:
+        raise SepError(f"sep_filter_batch failed: {_ERRNAMES.get(rc, rc)}: "
+                       f"{lib().sep_last_error(ctxs[0]._h).decode()}")
 
 
 class Context:

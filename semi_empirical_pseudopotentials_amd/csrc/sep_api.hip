@@ -546,83 +546,227 @@ int sep_sigma(sep_ctx *c, long m, double *sige) {
 }  // extern "C"
 
 // ---------------------------------------------------------------- filter
+// One filter pass (filter/filter.c:59-83 for all random states) in three
+// stages, so that several grids can be interleaved or captured into graphs:
+// start (allocations, uploads, zeroing), one Newton step per call (three
+// H.psi passes, plus the accumulation of a full ring block), finish.
 template <typename R>
-static int filter_impl(sep_ctx *c, const double *psi0, bool psi0_on_device, long ns,
-                       const double *an, const double *zn, long nc, long ms, double vmin,
-                       double dE) {
+struct FilterRun {
+  sep_ctx *c = nullptr;
+  long N = 0, ns = 0, nc = 0, ms = 0, K = 0, slot = 0, nvec = 0, j = 1;
+  double vmin = 0.0, dE_1 = 0.0;
+  std::vector<double> zn;
+  double *an_d = nullptr;
+  R *ring = nullptr;
+  bool sampled = false;
+};
+
+__global__ void k_zero(double *x, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+// zero fill by a kernel (also safe inside a captured graph)
+static int zero_dev(void *p, size_t bytes, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, static_cast<double *>(p),
+                     (long)(bytes / sizeof(double)));
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
+template <typename R>
+static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool psi0_on_device,
+                        long ns, const double *an, const double *zn, long nc, long ms, double vmin,
+                        double dE) {
   const long N = c->gd.N;
-  const double dE_1 = 4.0 / dE;
-  const long nvec = (ns + 1) / 2, S2 = 2 * nvec;
-  // ring of K Newton vectors: K * S2 * N reals
+  f.c = c;
+  f.N = N;
+  f.ns = ns;
+  f.nc = nc;
+  f.ms = ms;
+  f.vmin = vmin;
+  f.dE_1 = 4.0 / dE;
+  f.nvec = (ns + 1) / 2;
+  const long S2 = 2 * f.nvec;
   // K Newton vectors are accumulated per pass over the ring; the
   // accumulators move 2 ms 8 / K bytes per real-state point per step
   long K = nc < 256 ? nc : 256;
   while (K > 8 && (double)K * S2 * N * sizeof(R) > 64.0e9) K /= 2;
-  const long slot = S2 * N;
-  RCHK(ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * slot));
-  R *ring = reinterpret_cast<R *>(c->ring);
+  f.K = K;
+  f.slot = S2 * N;
+  RCHK(ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * f.slot));
+  RCHK(ensure(c, &c->tb, &c->tb_bytes, sizeof(cx<R>) * (size_t)f.nvec * N));
+  f.ring = reinterpret_cast<R *>(c->ring);
   RCHK(ensure_states(c, ns * ms, 1));
-  HIPCHK(c, hipMemsetAsync(c->states, 0, sizeof(double) * ns * ms * N, c->st));
+  RCHK(zero_dev(c->states, sizeof(double) * ns * ms * N, c->st));
   // slot 0 <- psi_0 (padding state zero)
-  HIPCHK(c, hipMemsetAsync(ring, 0, sizeof(R) * slot, c->st));
+  RCHK(zero_dev(f.ring, sizeof(R) * f.slot, c->st));
   const hipMemcpyKind kind = psi0_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   if (sizeof(R) == 8) {
-    HIPCHK(c, hipMemcpyAsync(ring, psi0, sizeof(double) * ns * N, kind, c->st));
+    HIPCHK(c, hipMemcpyAsync(f.ring, psi0, sizeof(double) * ns * N, kind, c->st));
   } else if (psi0_on_device) {
-    RCHK(convert_d2f(psi0, (float *)ring, ns * N, c->st));
+    RCHK(convert_d2f(psi0, (float *)f.ring, ns * N, c->st));
   } else {
     RCHK(ensure_states(c, ns, 2));
     HIPCHK(c, hipMemcpyAsync(c->states2, psi0, sizeof(double) * ns * N, kind, c->st));
-    RCHK(convert_d2f(c->states2, (float *)ring, ns * N, c->st));
+    RCHK(convert_d2f(c->states2, (float *)f.ring, ns * N, c->st));
   }
   // coefficients transposed to [j][ie] for the accumulation kernel
   std::vector<double> anT((size_t)ms * nc);
   for (long ie = 0; ie < ms; ie++)
     for (long j = 0; j < nc; j++) anT[j * ms + ie] = an[ie * nc + j];
-  double *an_d = nullptr;
-  HIPCHK(c, hipMalloc(&an_d, sizeof(double) * ms * nc));
-  HIPCHK(c, hipMemcpyAsync(an_d, anT.data(), sizeof(double) * ms * nc, hipMemcpyHostToDevice, c->st));
-  int rc = SEP_OK;
-  bool sampled = false;  // a step of the current ring block carries timing events
-  for (long j = 1; j < nc && rc == SEP_OK; j++) {
-    const long sp = (j - 1) % K, sj = j % K;
-    VecView<R> in{ring + sp * slot, ring + sp * slot + N, nullptr, 2 * N};
-    VecView<R> out{ring + sj * slot, ring + sj * slot + N, nullptr, 2 * N};
-    // hnorm.c:19-24: psi_j = dE_1 H psi_{j-1} - (2 + z_{j-1} + Vmin dE_1) psi_{j-1}
-    const double beta = 2.0 + zn[j - 1] + vmin * dE_1;
-    const bool sample = c->timing && sj == K / 2;
-    rc = apply_views<R>(c, in, out, (int)nvec, dE_1, beta, nullptr, sample ? c->tev : nullptr);
-    sampled |= sample;
-    if (rc == SEP_OK && (sj == K - 1 || j == nc - 1)) {
-      if (sampled) hipEventRecord(c->tev[4], c->st);
-      rc = accum_mfma_dev<R>(ring, slot, (int)(sj + 1), an_d, j - sj, (int)ms, c->states, (int)ns, N,
-                        c->st);
-      if (sampled && rc == SEP_OK) {
-        hipEventRecord(c->tev[5], c->st);
-        if (hipEventSynchronize(c->tev[5]) != hipSuccess) {
-          rc = SEP_ERR_HIP;
-          break;
-        }
-        float t;
-        for (int q = 0; q < 3; q++) {
-          hipEventElapsedTime(&t, c->tev[q], c->tev[q + 1]);
-          c->tsum[q] += t;
-        }
-        c->tcount++;
-        if (sj + 1 == K) {  // only full blocks time the accumulation of K vectors
-          hipEventElapsedTime(&t, c->tev[4], c->tev[5]);
-          c->tsum[3] += t;
-          c->tacc_count++;
-        }
+  HIPCHK(c, hipMalloc(&f.an_d, sizeof(double) * ms * nc));
+  HIPCHK(c, hipMemcpyAsync(f.an_d, anT.data(), sizeof(double) * ms * nc, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));  // anT outlives the copy
+  f.zn.assign(zn, zn + nc);
+  f.j = 1;
+  return SEP_OK;
+}
+
+// Newton step f.j (filter/filter.c:70-78, hnorm.c:19-24).  No host
+// synchronisation unless live timing is on (which graphs must not use).
+template <typename R>
+static int filter_step(FilterRun<R> &f) {
+  sep_ctx *c = f.c;
+  const long j = f.j, K = f.K, slot = f.slot, N = f.N;
+  const long sp = (j - 1) % K, sj = j % K;
+  VecView<R> in{f.ring + sp * slot, f.ring + sp * slot + N, nullptr, 2 * N};
+  VecView<R> out{f.ring + sj * slot, f.ring + sj * slot + N, nullptr, 2 * N};
+  // psi_j = dE_1 H psi_{j-1} - (2 + z_{j-1} + Vmin dE_1) psi_{j-1}
+  const double beta = 2.0 + f.zn[j - 1] + f.vmin * f.dE_1;
+  const bool sample = c->timing && sj == K / 2;
+  RCHK(apply_views<R>(c, in, out, (int)f.nvec, f.dE_1, beta, nullptr, sample ? c->tev : nullptr));
+  f.sampled |= sample;
+  if (sj == K - 1 || j == f.nc - 1) {
+    if (f.sampled) hipEventRecord(c->tev[4], c->st);
+    RCHK(accum_mfma_dev<R>(f.ring, slot, (int)(sj + 1), f.an_d, j - sj, (int)f.ms, c->states,
+                           (int)f.ns, N, c->st));
+    if (f.sampled) {
+      hipEventRecord(c->tev[5], c->st);
+      HIPCHK(c, hipEventSynchronize(c->tev[5]));
+      float t;
+      for (int q = 0; q < 3; q++) {
+        hipEventElapsedTime(&t, c->tev[q], c->tev[q + 1]);
+        c->tsum[q] += t;
       }
-      sampled = false;
+      c->tcount++;
+      if (sj + 1 == K) {  // only full blocks time the accumulation of K vectors
+        hipEventElapsedTime(&t, c->tev[4], c->tev[5]);
+        c->tsum[3] += t;
+        c->tacc_count++;
+      }
+    }
+    f.sampled = false;
+  }
+  f.j++;
+  return SEP_OK;
+}
+
+template <typename R>
+static int filter_finish(FilterRun<R> &f, int rc) {
+  sep_ctx *c = f.c;
+  if (rc == SEP_OK && f.nc == 1)
+    rc = accum_mfma_dev<R>(f.ring, f.slot, 1, f.an_d, 0, (int)f.ms, c->states, (int)f.ns, f.N,
+                           c->st);
+  hipStreamSynchronize(c->st);
+  if (f.an_d) hipFree(f.an_d);
+  f.an_d = nullptr;
+  if (rc != SEP_OK) return rc;
+  return normalize_dev(c, c->states, f.ns * f.ms);  // filter.c:33
+}
+
+template <typename R>
+static int filter_impl(sep_ctx *c, const double *psi0, bool psi0_on_device, long ns,
+                       const double *an, const double *zn, long nc, long ms, double vmin,
+                       double dE) {
+  FilterRun<R> f;
+  int rc = filter_start<R>(f, c, psi0, psi0_on_device, ns, an, zn, nc, ms, vmin, dE);
+  while (rc == SEP_OK && f.j < nc) rc = filter_step<R>(f);
+  return filter_finish<R>(f, rc);
+}
+
+// ------------------------------------------------- batched uneven grids
+template <typename R>
+static int filter_batch_impl(int nd, sep_ctx *const *ctxs, const void *const *psi0,
+                             const long *ns, const double *const *an, const double *const *zn,
+                             const long *nc, const long *ms, const double *vmin, const double *dE,
+                             int mode) {
+  std::vector<FilterRun<R>> runs(nd);
+  std::vector<hipGraphExec_t> execs(nd, nullptr);
+  std::vector<hipGraph_t> graphs(nd, nullptr);
+  int rc = SEP_OK;
+  for (int d = 0; d < nd && rc == SEP_OK; d++) {
+    rc = filter_start<R>(runs[d], ctxs[d], static_cast<const double *>(psi0[d]), true, ns[d],
+                         an[d], zn[d], nc[d], ms[d], vmin[d], dE[d]);
+    if (rc == SEP_OK) rc = hpsi_prepare<R>(ctxs[d]->gd);
+  }
+  if (rc == SEP_OK && mode == SEP_BATCH_GRAPH) {
+    // each grid's whole step sequence as one graph on its own stream
+    for (int d = 0; d < nd && rc == SEP_OK; d++) {
+      sep_ctx *c = ctxs[d];
+      const bool timing = c->timing;
+      c->timing = false;
+      if (hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = fail(c, SEP_ERR_HIP, "batch: stream capture failed to start");
+      } else {
+        while (rc == SEP_OK && runs[d].j < nc[d]) rc = filter_step<R>(runs[d]);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->st, &g);
+        graphs[d] = g;
+        if (rc == SEP_OK && e != hipSuccess) rc = fail(c, SEP_ERR_HIP, "batch: capture failed");
+        if (rc == SEP_OK && hipGraphInstantiate(&execs[d], g, nullptr, nullptr, 0) != hipSuccess)
+          rc = fail(c, SEP_ERR_HIP, "batch: graph instantiation failed");
+      }
+      c->timing = timing;
+    }
+    for (int d = 0; d < nd && rc == SEP_OK; d++)
+      if (hipGraphLaunch(execs[d], ctxs[d]->st) != hipSuccess)
+        rc = fail(ctxs[d], SEP_ERR_HIP, "batch: graph launch failed");
+  } else if (rc == SEP_OK) {
+    // interleave: one Newton step of every unfinished grid per round, each
+    // grid on its own stream, so that small grids fill what large ones leave
+    bool left = true;
+    while (rc == SEP_OK && left) {
+      left = false;
+      for (int d = 0; d < nd && rc == SEP_OK; d++)
+        if (runs[d].j < nc[d]) {
+          rc = filter_step<R>(runs[d]);
+          left = true;
+        }
     }
   }
-  if (rc == SEP_OK && nc == 1) rc = accum_mfma_dev<R>(ring, slot, 1, an_d, 0, (int)ms, c->states, (int)ns, N, c->st);
-  hipStreamSynchronize(c->st);  // anT must outlive the async copy
-  hipFree(an_d);
-  if (rc != SEP_OK) return rc;
-  return normalize_dev(c, c->states, ns * ms);  // filter.c:33
+  int rc2 = SEP_OK;
+  for (int d = 0; d < nd; d++) {
+    const int r = filter_finish<R>(runs[d], rc);
+    if (rc2 == SEP_OK) rc2 = r;
+    if (execs[d]) hipGraphExecDestroy(execs[d]);
+    if (graphs[d]) hipGraphDestroy(graphs[d]);
+  }
+  return rc != SEP_OK ? rc : rc2;
+}
+
+extern "C" int sep_filter_batch(int ndots, sep_ctx *const *ctxs, const void *const *psi0_dev,
+                                const long *ns, const double *const *an,
+                                const double *const *zn, const long *nc, const long *ms,
+                                const double *vmin, const double *dE, int precision, int mode) {
+  if (ndots < 1 || !ctxs || !psi0_dev || !ns || !an || !zn || !nc || !ms || !vmin || !dE)
+    return SEP_ERR_ARG;
+  if (mode != SEP_BATCH_INTERLEAVE && mode != SEP_BATCH_GRAPH) return SEP_ERR_ARG;
+  for (int d = 0; d < ndots; d++) {
+    sep_ctx *c = ctxs[d];
+    if (!c || !psi0_dev[d] || ns[d] < 1 || !an[d] || !zn[d] || nc[d] < 1 || ms[d] < 1 ||
+        !(dE[d] > 0.0))
+      return SEP_ERR_ARG;
+    if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+    for (int e = 0; e < d; e++)
+      if (ctxs[e] == c) return fail(c, SEP_ERR_ARG, "batch: context %d appears twice", d);
+    if (c->device != ctxs[0]->device)
+      return fail(c, SEP_ERR_ARG, "batch: all contexts must be on one device");
+  }
+  HIPCHK(ctxs[0], hipSetDevice(ctxs[0]->device));
+  if (precision == SEP_PREC_FP64)
+    return filter_batch_impl<double>(ndots, ctxs, psi0_dev, ns, an, zn, nc, ms, vmin, dE, mode);
+  if (precision == SEP_PREC_FP32)
+    return filter_batch_impl<float>(ndots, ctxs, psi0_dev, ns, an, zn, nc, ms, vmin, dE, mode);
+  return fail(ctxs[0], SEP_ERR_ARG, "unknown precision %d", precision);
 }
 
 extern "C" int sep_filter(sep_ctx *c, const double *psi0, long ns, const double *an,

@@ -824,6 +824,7 @@ static void plane_fwd_launch(VecView<R> in, cx<R> *tb, long N, int nz, int nvec,
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       attr = true;
     }
+    if (nvec == 0) return;  // attribute only (before graph capture)
     hipLaunchKernelGGL((k_plane_fwd_w<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, in,
                        tb, N, nz, tw);
   } else {
@@ -832,6 +833,7 @@ static void plane_fwd_launch(VecView<R> in, cx<R> *tb, long N, int nz, int nvec,
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       attr = true;
     }
+    if (nvec == 0) return;  // attribute only (before graph capture)
     hipLaunchKernelGGL((k_plane_fwd<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, in,
                        tb, N, nz, tw);
   }
@@ -849,6 +851,7 @@ static void plane_bwd_launch(const cx<R> *tb, VecView<R> pin, VecView<R> out, Ep
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       attr = true;
     }
+    if (nvec == 0) return;  // attribute only (before graph capture)
     hipLaunchKernelGGL((k_plane_bwd_w<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st,
                        tb, pin, out, epi, N, nz, tw);
   } else {
@@ -857,6 +860,7 @@ static void plane_bwd_launch(const cx<R> *tb, VecView<R> pin, VecView<R> out, Ep
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       attr = true;
     }
+    if (nvec == 0) return;  // attribute only (before graph capture)
     hipLaunchKernelGGL((k_plane_bwd<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st, tb,
                        pin, out, epi, N, nz, tw);
   }
@@ -1005,6 +1009,22 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
   if (ev) hipEventRecord(ev[3], st);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
+
+// Kernel attributes of this grid's H.psi kernels, set outside any stream
+// capture (hipFuncSetAttribute is not a stream operation).
+template <typename R>
+int hpsi_prepare(const GridDev &gd) {
+  const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
+  if (gd.force_lines || !plane_supported(gd.nx, gd.ny, prec)) return SEP_OK;
+  const cx<R> *tw = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
+  VecView<R> none{nullptr, nullptr, nullptr, 0};
+  Epi<R> epi{nullptr, 0.0, 0.0, nullptr};
+  HOps<R>::plane_fwd(gd.nx, none, nullptr, gd.N, gd.nz, 0, tw, nullptr);
+  HOps<R>::plane_bwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr, nullptr);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+template int hpsi_prepare<double>(const GridDev &);
+template int hpsi_prepare<float>(const GridDev &);
 
 template int hpsi_apply<double>(const GridDev &, const VecView<double> &, const VecView<double> &,
                                 cx<double> *, const double *, int, double, double, double *,

@@ -23,6 +23,8 @@ bool hop_supported(int n);
 bool plane_supported(int nx, int ny, int prec);
 
 template <typename R>
+int hpsi_prepare(const GridDev &gd);
+template <typename R>
 int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
                const R *pot, int nvec, double alpha, double beta, double *part, long *nblk,
                hipStream_t st, hipEvent_t *ev = nullptr);
