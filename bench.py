@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=24)
+    p.add_argument("--no-batched", action="store_true",
+                   help="skip the batched uneven-grid measurement (8 dots, 64-160^3)")
     p.add_argument("--no-check", action="store_true",
                    help="skip the FP32-pipeline accuracy check (2 nm dot, 64^3)")
     p.add_argument("--kernels-only", action="store_true",
@@ -120,6 +122,77 @@ def precision_check(sep, decks, read_deck, np):
             "edge_levels": int(edge.sum()), "edge_energies_Ha": [round(float(e), 8) for e in ev64[edge]],
             "abs_err_meV": [round(e, 6) for e in err], "max_abs_err_meV": round(max(err), 6),
             "pipeline_s": {"fp64": round(res["fp64"][3], 3), "fp32": round(res["fp32"][3], 3)}}
+
+
+def batched_uneven(sep, decks, read_deck, np, torch, steps=1):
+    """SURVEY config 4: 8 CdSe dots, diameters seeded-uniform in 2-5 nm, each
+    on the smallest of 64/96/128/160^3 with spacing <= 0.85 bohr, Newton
+    length 2048, 16 targets x 8 random states each, complex64 filter.  Times
+    one filter pass of all dots: one dot after another, interleaved on their
+    streams, and as one captured graph per dot."""
+    import tempfile
+    rng = np.random.default_rng(2024)
+    diams = np.sort(rng.uniform(2.0, 5.0, 8))
+    nc, ms, ns = 2048, 16, 8
+    dots = []
+    for k, dnm in enumerate(diams):
+        syms, xyz = decks.cdse_dot(float(dnm))
+        n = 160
+        for cand in (64, 96, 128, 160):
+            half, dsp = decks.box_from_atoms(xyz, cand, cand, cand)
+            if dsp.max() <= 0.85:
+                n = cand
+                break
+        tmp = tempfile.mkdtemp(prefix="sep_c4_")
+        c, h = decks.WINDOW_CENTER_HA, decks.WINDOW_HALF_HA
+        decks.write_deck(tmp, syms, xyz, n, n, n, ms, ns, nc, c - h, c + h)
+        d = read_deck(tmp)
+        ctx = sep.Context(n, n, n, d["dx"], d["dy"], d["dz"], d["xmin"], d["ymin"], d["zmin"])
+        ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
+                            expot=d["expot"])
+        tune = ctx.autotune(sep.SEP_PREC_FP32, (ns + 1) // 2)
+        dE, _, _ = ctx.energy_range()
+        zn, _ = sep.leja_points(nc)
+        an = sep.newton_coeffs(zn, np.linspace(d["Elmin"], d["Elmax"], ms), dE, ctx.vmin,
+                               (nc / (2.5 * dE)) ** 2)
+        N = n ** 3
+        psi0 = np.empty((ns, N))
+        idum, r = -2001 - k, None
+        for j in range(ns):
+            psi0[j], idum, r = sep.init_psi(idum, N, d["dv"], r)
+        dots.append(dict(ctx=ctx, n=n, d_nm=round(float(dnm), 3), natom=d["natom"],
+                         psi0=torch.from_numpy(psi0).to("cuda:0"), an=an, zn=zn, dE=dE,
+                         vmin=ctx.vmin, tune_ms=[round(t, 4) for t in tune]))
+        del psi0
+    hpsi = sum(ns * (nc - 1) for _ in dots) * steps
+
+    def seq():
+        for o in dots:
+            o["ctx"].filter_dev(o["psi0"].data_ptr(), ns, o["an"], o["zn"], o["vmin"], o["dE"],
+                                sep.SEP_PREC_FP32)
+
+    def bat(mode):
+        sep.filter_batch([o["ctx"] for o in dots], [o["psi0"].data_ptr() for o in dots],
+                         [ns] * len(dots), [o["an"] for o in dots], [o["zn"] for o in dots],
+                         [o["vmin"] for o in dots], [o["dE"] for o in dots],
+                         sep.SEP_PREC_FP32, mode)
+
+    out = {"dots": [{"d_nm": o["d_nm"], "grid": o["n"], "atoms": o["natom"],
+                     "autotune_ms_plane_lines": o["tune_ms"]} for o in dots]}
+    for name, fn in (("sequential", seq), ("interleaved", lambda: bat(sep.SEP_BATCH_INTERLEAVE)),
+                     ("graph", lambda: bat(sep.SEP_BATCH_GRAPH))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        out[name] = {"s": round(t / steps, 3), "hpsi_per_s": round(hpsi / t, 1)}
+        log(f"config 4 {name}: {t / steps:.3f} s, {hpsi / t:.0f} H.psi/s")
+    for o in dots:
+        o["ctx"].close()
+    return out
 
 
 def main():
@@ -280,6 +353,14 @@ def main():
                          f"x {ms} targets at {n}^3, {tc:.1f} s"}
         log(f"cpu oracle: {steps / tc:.2f} H.psi/s on {threads} threads")
 
+    batched = None
+    if not args.no_batched:
+        ctx.close()  # free the 128^3 workload's device memory first
+        ctx = None
+        del psi0_dev
+        torch.cuda.empty_cache()
+        batched = batched_uneven(sep, decks, read_deck, np, torch)
+
     res = {
         "metric": METRIC,
         "value": round(v32, 1),
@@ -306,11 +387,13 @@ def main():
         "accum_ms_per_block": {"fp32": round(live32[3], 4),
                                "fp64": round(live64[3], 4) if live64 is not None else None},
         "fp32_pipeline_check": check,
+        "batched_uneven": batched,
         "hf_adjoint": dict(adj, atoms=d["natom"], states=m),
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
-    ctx.close()
+    if ctx is not None:
+        ctx.close()
 
 
 if __name__ == "__main__":

@@ -90,6 +90,7 @@ SIGNATURES = {
     "sep_filter_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D, _I]),
     "sep_bench_hpsi": (_I, [_V, _I, _L, _L, _L, _dp]),
     "sep_set_timing": (_I, [_V, _I]),
+    "sep_autotune": (_I, [_V, _I, _L, _dp, _dp]),
     "sep_filter_batch": (_I, [_I, _V, _V, _lp, _V, _V, _lp, _lp, _dp, _dp, _I, _I]),
     "sep_get_timing": (_I, [_V, _dp, ctypes.POINTER(ctypes.c_long)]),
     "sep_gram_dev": (_I, [_V, _I, _V, _L, _V, _L, _dp]),
@@ -369,6 +370,12 @@ class Context:
         rc = lib().sep_bench_hpsi(self._h, precision, nvec, ms, iters, _p(out))
         self._chk(rc, "sep_bench_hpsi")
         return out
+
+    def autotune(self, precision, nvec=8):
+        """Pick the faster H.psi kernel family for this grid (sep_autotune)."""
+        tp, tl = np.zeros(1), np.zeros(1)
+        self._chk(lib().sep_autotune(self._h, precision, nvec, _p(tp), _p(tl)), "sep_autotune")
+        return float(tp[0]), float(tl[0])
 
     def set_timing(self, on):
         """Sample kernel durations inside the filter loop (see sep_set_timing)."""

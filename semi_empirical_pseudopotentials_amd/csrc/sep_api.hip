@@ -200,7 +200,7 @@ int sep_open(const sep_grid_desc *g, sep_ctx **out) {
   c->gd.kyy = tabs_d[1];
   c->gd.kzz = tabs_d[2];
   c->gd.ekin = c->ekinmax * (ny_1 * nx_1 * nz_1);
-  c->gd.force_lines = getenv("SEP_FORCE_LINE_KERNELS") != nullptr;
+  c->gd.force_lines[0] = c->gd.force_lines[1] = getenv("SEP_FORCE_LINE_KERNELS") != nullptr;
   *out = c;
   return SEP_OK;
 }
@@ -1167,6 +1167,42 @@ extern "C" int sep_bench_hpsi(sep_ctx *c, int precision, long nvec, long ms, lon
   if (precision == SEP_PREC_FP64) return bench_impl<double>(c, nvec, ms, iters, kernel_ms);
   if (precision == SEP_PREC_FP32) return bench_impl<float>(c, nvec, ms, iters, kernel_ms);
   return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
+}
+
+// Kernel-family choice per grid and precision: time passes A+B+C of the
+// plane kernels and of the row + strided-line kernels on nvec resident
+// random vectors (bench_impl) and keep the faster.
+extern "C" int sep_autotune(sep_ctx *c, int precision, long nvec, double *ms_plane,
+                            double *ms_lines) {
+  if (!c || nvec < 1 || (precision != SEP_PREC_FP64 && precision != SEP_PREC_FP32))
+    return SEP_ERR_ARG;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  double k[5];
+  double tp = -1.0, tl = -1.0;
+  const bool had = c->gd.force_lines[precision];
+  if (plane_supported(c->gd.nx, c->gd.ny, precision)) {
+    c->gd.force_lines[precision] = false;
+    const int rc = precision == SEP_PREC_FP64 ? bench_impl<double>(c, nvec, 1, 5, k)
+                                              : bench_impl<float>(c, nvec, 1, 5, k);
+    if (rc != SEP_OK) {
+      c->gd.force_lines[precision] = had;
+      return rc;
+    }
+    tp = k[4];
+  }
+  c->gd.force_lines[precision] = true;
+  const int rc = precision == SEP_PREC_FP64 ? bench_impl<double>(c, nvec, 1, 5, k)
+                                            : bench_impl<float>(c, nvec, 1, 5, k);
+  if (rc != SEP_OK) {
+    c->gd.force_lines[precision] = had;
+    return rc;
+  }
+  tl = k[4];
+  c->gd.force_lines[precision] = !(tp > 0.0 && tp <= tl);
+  if (ms_plane) *ms_plane = tp;
+  if (ms_lines) *ms_lines = tl;
+  return SEP_OK;
 }
 
 extern "C" int sep_set_timing(sep_ctx *c, int on) {

@@ -978,7 +978,7 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
   const cx<R> *twy = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[1] : gd.tw32[1]);
   const cx<R> *twz = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[2] : gd.tw32[2]);
   const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
-  const bool plane = gd.force_lines ? false : plane_supported(nx, ny, prec);
+  const bool plane = gd.force_lines[prec] ? false : plane_supported(nx, ny, prec);
   if (ev) hipEventRecord(ev[0], st);
   // pass A
   if (plane) {
@@ -1015,7 +1015,7 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
 template <typename R>
 int hpsi_prepare(const GridDev &gd) {
   const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
-  if (gd.force_lines || !plane_supported(gd.nx, gd.ny, prec)) return SEP_OK;
+  if (gd.force_lines[prec] || !plane_supported(gd.nx, gd.ny, prec)) return SEP_OK;
   const cx<R> *tw = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
   VecView<R> none{nullptr, nullptr, nullptr, 0};
   Epi<R> epi{nullptr, 0.0, 0.0, nullptr};

@@ -16,7 +16,10 @@ struct GridDev {
   void *tw32[3];  // same, complex64
   double *kxx, *kyy, *kzz;  // filter/init.c:125-135 axis tables (1/N folded in)
   double ekin;              // Ekinmax / N (filter/init.c:137)
-  bool force_lines;         // test hook: use the row/line kernels even when a plane fits
+  // per precision (SEP_PREC_FP64 = 0, SEP_PREC_FP32 = 1): use the row/line
+  // kernels even where a plane kernel exists (sep_autotune, or the test hook
+  // SEP_FORCE_LINE_KERNELS)
+  bool force_lines[2];
 };
 
 bool hop_supported(int n);

@@ -57,3 +57,22 @@ def test_batch_equals_single(sep, dots, prec, mode):
     for o, ref in zip(dots, single):
         got = o["ctx"].states_download(o["ns"] * o["an"].shape[0])
         assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_autotune_choice_keeps_results(sep, dots, prec):
+    """The autotuner only picks between two correct kernel families: H.psi after
+    it agrees with H.psi before it (plane vs row + line kernels)."""
+    P = sep.SEP_PREC_FP32 if prec == "fp32" else sep.SEP_PREC_FP64
+    o = dots[2]
+    ctx = o["ctx"]
+    N = ctx.N
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((2, N)) + 1j * rng.standard_normal((2, N))
+    before = ctx.apply_h(x) if P == sep.SEP_PREC_FP64 else None
+    tp, tl = ctx.autotune(P, 2)
+    print("autotune", prec, tp, tl, flush=True)
+    assert tl > 0.0 and (tp > 0.0 or tp == -1.0)
+    if before is not None:
+        after = ctx.apply_h(x)
+        assert np.abs(after - before).max() <= 1e-12 * np.abs(before).max()
