@@ -50,10 +50,12 @@ __device__ __forceinline__ cx<R> epilogue(const Epi<R> &e, cx<R> t, cx<R> p, R V
   cx<R> o;
   o.re = a * (t.re + V * p.re) - b * p.re;
   o.im = a * (t.im + V * p.im) - b * p.im;
-  s[0] += (double)p.re * (double)o.re;
-  s[1] += (double)p.im * (double)o.im;
-  s[2] += (double)o.re * (double)o.re;
-  s[3] += (double)o.im * (double)o.im;
+  if (e.part) {  // uniform: the FP64 sums only when a reduction was asked for
+    s[0] += (double)p.re * (double)o.re;
+    s[1] += (double)p.im * (double)o.im;
+    s[2] += (double)o.re * (double)o.re;
+    s[3] += (double)o.im * (double)o.im;
+  }
   return o;
 }
 
