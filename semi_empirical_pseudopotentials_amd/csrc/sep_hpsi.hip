@@ -630,19 +630,19 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
     const int row = r * RX + tl;
     const bool act = row < NX;
     const int rs = act ? row : 0;
-    // psi_{j-1} and V of the row: loaded before the FFT (in flight during
-    // it) at 512 threads; after it at 1024, where the register budget is 128
-    // and the other waves hide the latency
+    // psi_{j-1} and V of the row are loaded before the FFT (in flight
+    // during it); at 1024 threads (register budget 128) only V is, psi_{j-1}
+    // after the FFT
     constexpr bool early = T < 1024;
     cx<R> p[E];
     R V[E];
-    if (early && act) {
+    if (act) {
 #pragma unroll
       for (int b = 0; b < E / rl_; b++)
 #pragma unroll
         for (int k = 0; k < rl_; k++) {
           const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
-          p[b * rl_ + k] = pin.ld(vec, gi);
+          if (early) p[b * rl_ + k] = pin.ld(vec, gi);
           V[b * rl_ + k] = epi.pot[gi];
         }
     }
@@ -666,7 +666,6 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
           for (int k = 0; k < rl_; k++) {
             const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
             p[b * rl_ + k] = pin.ld(vec, gi);
-            V[b * rl_ + k] = epi.pot[gi];
           }
       }
 #pragma unroll
