@@ -587,9 +587,14 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
   f.nvec = (ns + 1) / 2;
   const long S2 = 2 * f.nvec;
   // K Newton vectors are accumulated per pass over the ring; the
-  // accumulators move 2 ms 8 / K bytes per real-state point per step
-  long K = nc < 256 ? nc : 256;
-  while (K > 8 && (double)K * S2 * N * sizeof(R) > 64.0e9) K /= 2;
+  // accumulators move 2 ms 8 / K bytes per real-state point per step.  The
+  // ring takes up to 96 GB and half of the free device memory (the ring
+  // already held by this context counts as free).
+  size_t mfree = 0, mtotal = 0;
+  if (hipMemGetInfo(&mfree, &mtotal) != hipSuccess) mfree = 0;
+  const double cap = std::min(96.0e9, 0.5 * ((double)mfree + (double)c->ring_bytes));
+  long K = nc < 512 ? nc : 512;
+  while (K > 8 && (double)K * S2 * N * sizeof(R) > cap) K /= 2;
   f.K = K;
   f.slot = S2 * N;
   RCHK(ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * f.slot));
