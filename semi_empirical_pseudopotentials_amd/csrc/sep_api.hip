@@ -596,23 +596,30 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
   long K = nc < 512 ? nc : 512;
   while (K > 8 && (double)K * S2 * N * sizeof(R) > cap) K /= 2;
   f.K = K;
-  f.slot = S2 * N;
-  RCHK(ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * f.slot));
+  // ring layout [state][Newton vector][point]: row (s, k) at s * K * N + k * N,
+  // so the accumulation walks each state's K rows within K * N elements
+  f.slot = N;
+  RCHK(ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * S2 * N));
   RCHK(ensure(c, &c->tb, &c->tb_bytes, sizeof(cx<R>) * (size_t)f.nvec * N));
   f.ring = reinterpret_cast<R *>(c->ring);
   RCHK(ensure_states(c, ns * ms, 1));
   RCHK(zero_dev(c->states, sizeof(double) * ns * ms * N, c->st));
-  // slot 0 <- psi_0 (padding state zero)
-  RCHK(zero_dev(f.ring, sizeof(R) * f.slot, c->st));
+  // row (s, 0) <- psi_0 of state s; the padding state (odd ns) starts at zero
+  const long ss = K * N;  // state stride of the ring
+  if (ns < S2) RCHK(zero_dev(f.ring + ns * ss, sizeof(R) * N, c->st));
   const hipMemcpyKind kind = psi0_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   if (sizeof(R) == 8) {
-    HIPCHK(c, hipMemcpyAsync(f.ring, psi0, sizeof(double) * ns * N, kind, c->st));
-  } else if (psi0_on_device) {
-    RCHK(convert_d2f(psi0, (float *)f.ring, ns * N, c->st));
+    HIPCHK(c, hipMemcpy2DAsync(f.ring, sizeof(R) * ss, psi0, sizeof(double) * N,
+                               sizeof(double) * N, ns, kind, c->st));
   } else {
-    RCHK(ensure_states(c, ns, 2));
-    HIPCHK(c, hipMemcpyAsync(c->states2, psi0, sizeof(double) * ns * N, kind, c->st));
-    RCHK(convert_d2f(c->states2, (float *)f.ring, ns * N, c->st));
+    const double *src = psi0;
+    if (!psi0_on_device) {
+      RCHK(ensure_states(c, ns, 2));
+      HIPCHK(c, hipMemcpyAsync(c->states2, psi0, sizeof(double) * ns * N, kind, c->st));
+      src = c->states2;
+    }
+    for (long st = 0; st < ns; st++)
+      RCHK(convert_d2f(src + st * N, (float *)f.ring + st * ss, N, c->st));
   }
   // coefficients transposed to [j][ie] for the accumulation kernel
   std::vector<double> anT((size_t)ms * nc);
@@ -633,8 +640,9 @@ static int filter_step(FilterRun<R> &f) {
   sep_ctx *c = f.c;
   const long j = f.j, K = f.K, slot = f.slot, N = f.N;
   const long sp = (j - 1) % K, sj = j % K;
-  VecView<R> in{f.ring + sp * slot, f.ring + sp * slot + N, nullptr, 2 * N};
-  VecView<R> out{f.ring + sj * slot, f.ring + sj * slot + N, nullptr, 2 * N};
+  const long ss = K * N;  // state stride: vector v = states (2v, 2v+1)
+  VecView<R> in{f.ring + sp * slot, f.ring + ss + sp * slot, nullptr, 2 * ss};
+  VecView<R> out{f.ring + sj * slot, f.ring + ss + sj * slot, nullptr, 2 * ss};
   // psi_j = dE_1 H psi_{j-1} - (2 + z_{j-1} + Vmin dE_1) psi_{j-1}
   const double beta = 2.0 + f.zn[j - 1] + f.vmin * f.dE_1;
   const bool sample = c->timing && sj == K / 2;
@@ -642,8 +650,8 @@ static int filter_step(FilterRun<R> &f) {
   f.sampled |= sample;
   if (sj == K - 1 || j == f.nc - 1) {
     if (f.sampled) hipEventRecord(c->tev[4], c->st);
-    RCHK(accum_mfma_dev<R>(f.ring, slot, (int)(sj + 1), f.an_d, j - sj, (int)f.ms, c->states,
-                           (int)f.ns, N, c->st));
+    RCHK(accum_mfma_dev<R>(f.ring, slot, ss, (int)(sj + 1), f.an_d, j - sj, (int)f.ms,
+                           c->states, (int)f.ns, N, c->st));
     if (f.sampled) {
       hipEventRecord(c->tev[5], c->st);
       HIPCHK(c, hipEventSynchronize(c->tev[5]));
@@ -669,7 +677,8 @@ template <typename R>
 static int filter_finish(FilterRun<R> &f, int rc) {
   sep_ctx *c = f.c;
   if (rc == SEP_OK && f.nc == 1)
-    rc = accum_mfma_dev<R>(f.ring, f.slot, 1, f.an_d, 0, (int)f.ms, c->states, (int)f.ns, f.N,
+    rc = accum_mfma_dev<R>(f.ring, f.slot, f.K * f.N, 1, f.an_d, 0, (int)f.ms, c->states,
+                           (int)f.ns, f.N,
                            c->st);
   hipStreamSynchronize(c->st);
   if (f.an_d) hipFree(f.an_d);
@@ -1144,7 +1153,8 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
     rc = hpsi_apply<R>(c->gd, in, out, tb, pot_of<R>(c), (int)nvec, 0.3, 1.7, nullptr, nullptr,
                        c->st, ev);
     if (rc == SEP_OK)
-      rc = accum_mfma_dev<R>(ring, S2 * N, (int)K, an_d, 0, (int)ms, c->states, (int)S2, N, c->st);
+      rc = accum_mfma_dev<R>(ring, S2 * N, N, (int)K, an_d, 0, (int)ms, c->states, (int)S2, N,
+                             c->st);
     hipEventRecord(ev[4], c->st);
     if (hipEventSynchronize(ev[4]) != hipSuccess) rc = SEP_ERR_HIP;
     if (it >= 0 && rc == SEP_OK) {

@@ -226,7 +226,8 @@ template int combine_dev<float>(const float *, long, const double *, long, long,
 // (FP64 for the FP64 filter, FP32 for the complex64 filter) and added to the
 // FP64 accumulators.
 template <typename R>
-__global__ void __launch_bounds__(256) k_accum_mfma(const R *__restrict__ ring, long slot, int cnt,
+__global__ void __launch_bounds__(256) k_accum_mfma(const R *__restrict__ ring, long slot,
+                                                    long sstride, int cnt,
                                                     const double *__restrict__ anT, long j0,
                                                     int ms, double *__restrict__ acc, long N) {
   const long s = blockIdx.y;
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(256) k_accum_mfma(const R *__restrict__ ring, 
   const long g0 = (long)blockIdx.x * 256 + w * 64;
   const int ie0 = blockIdx.z * 32;
   const int kq = lane >> 4, li = lane & 15;
-  const R *rp = ring + s * N;
+  const R *rp = ring + s * sstride;
   typename Mfma<R>::acc_t d[2][4];
 #pragma unroll
   for (int x = 0; x < 2; x++)
@@ -280,7 +281,7 @@ __global__ void __launch_bounds__(256) k_accum_mfma(const R *__restrict__ ring, 
 // Needs N % 256 == 0 (every block full).
 template <typename R>
 __global__ void __launch_bounds__(256) k_accum_mfma_v(const R *__restrict__ ring, long slot,
-                                                      int cnt, const double *__restrict__ anT,
+                                                      long sstride, int cnt, const double *__restrict__ anT,
                                                       long j0, int ms, double *__restrict__ acc,
                                                       long N) {
   constexpr int VW = 16 / sizeof(R);
@@ -291,7 +292,7 @@ __global__ void __launch_bounds__(256) k_accum_mfma_v(const R *__restrict__ ring
   const long g0 = (long)blockIdx.x * 256 + w * 64;
   const int ie0 = blockIdx.z * 32;
   const int kq = lane >> 4, li = lane & 15;
-  const R *rp = ring + s * N + g0 + VW * li;
+  const R *rp = ring + s * sstride + g0 + VW * li;
   typename Mfma<R>::acc_t d[2][4];
 #pragma unroll
   for (int x = 0; x < 2; x++)
@@ -357,20 +358,22 @@ __global__ void __launch_bounds__(256) k_accum_mfma_v(const R *__restrict__ ring
 }
 
 template <typename R>
-int accum_mfma_dev(const R *ring, long slot, int cnt, const double *anT_d, long j0, int ms,
-                   double *acc, int ns, long N, hipStream_t st) {
+int accum_mfma_dev(const R *ring, long slot, long sstride, int cnt, const double *anT_d, long j0,
+                   int ms, double *acc, int ns, long N, hipStream_t st) {
   dim3 grid((unsigned)((N + 255) / 256), (unsigned)ns, (unsigned)((ms + 31) / 32));
-  if (N % 256 == 0 && slot % 4 == 0)
-    hipLaunchKernelGGL(k_accum_mfma_v<R>, grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms,
-                       acc, N);
+  if (N % 256 == 0 && slot % 4 == 0 && sstride % 4 == 0)
+    hipLaunchKernelGGL(k_accum_mfma_v<R>, grid, dim3(256), 0, st, ring, slot, sstride, cnt, anT_d,
+                       j0, ms, acc, N);
   else
-    hipLaunchKernelGGL(k_accum_mfma<R>, grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms,
-                       acc, N);
+    hipLaunchKernelGGL(k_accum_mfma<R>, grid, dim3(256), 0, st, ring, slot, sstride, cnt, anT_d, j0,
+                       ms, acc, N);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
-template int accum_mfma_dev<double>(const double *, long, int, const double *, long, int, double *,
+template int accum_mfma_dev<double>(const double *, long, long, int, const double *, long, int,
+                                    double *,
                                     int, long, hipStream_t);
-template int accum_mfma_dev<float>(const float *, long, int, const double *, long, int, double *,
+template int accum_mfma_dev<float>(const float *, long, long, int, const double *, long, int,
+                                   double *,
                                    int, long, hipStream_t);
 
 // ---------------------------------------------------- double-double Gram

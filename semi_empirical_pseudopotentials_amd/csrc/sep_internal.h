@@ -55,20 +55,19 @@ int hf_adjoint_dev(const GridDev &gd, double xmin, double ymin, double zmin, dou
                    int np, int nsplit, double *work, double *htab, double *dR, double *dscale,
                    hipStream_t st);
 
+
 int sub_scaled_rows_dev(double *y, long ldy, const double *x, long ldx, long m, long n,
                         const double *s_d, hipStream_t st);
 int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStream_t st);
 int convert_d2f(const double *in, float *out, long n, hipStream_t st);
 int convert_f2d(const float *in, double *out, long n, hipStream_t st);
-// acc[(s*ms + ie)*N + g] += sum_{k<cnt} anT[(j0 + k)*ms + ie] * ring[k*slot + s*N + g]
-template <typename R>
-int accum_dev(const R *ring, long slot, int cnt, const double *anT_d, long j0, int ms,
-              double *acc, int ns, long N, hipStream_t st);
 
-// same contraction on the matrix cores (sep_gemm.hip)
+// Newton accumulation on the matrix cores (sep_gemm.hip):
+// acc[(s*ms + ie)*N + g] += sum_{k<cnt} anT[(j0 + k)*ms + ie] * ring row (s, k)
+// ring row (state s, Newton vector k) at ring + s * sstride + k * slot
 template <typename R>
-int accum_mfma_dev(const R *ring, long slot, int cnt, const double *anT_d, long j0, int ms,
-                   double *acc, int ns, long N, hipStream_t st);
+int accum_mfma_dev(const R *ring, long slot, long sstride, int cnt, const double *anT_d, long j0,
+                   int ms, double *acc, int ns, long N, hipStream_t st);
 
 // ---- GEMMs (sep_gemm.hip); all operands state-major (row i = one state)
 // C[i][j] = sum_g A[i][g] B[j][g]   (m x mb), deterministic split-K

@@ -1,6 +1,5 @@
 // sep_kernels.hip -- non-FFT kernels of libsep: V(r) assembly, deterministic
-// per-state reductions, the K-blocked Newton accumulation and the
-// double-double Gram used to decide the SVD cutoff.
+// per-state reductions, row scaling and precision conversion.
 #include <hip/hip_runtime.h>
 
 #include "fft_core.h"
@@ -175,66 +174,5 @@ int convert_f2d(const float *in, double *out, long n, hipStream_t st) {
   hipLaunchKernelGGL(k_f2d, dim3(4096), dim3(256), 0, st, in, out, n);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
-
-// --------------------------------------------- K-blocked Newton accumulation
-// filter/filter.c:74-78 applied to `cnt` stored Newton vectors at once:
-//   acc[s][ie][g] += sum_k anT[j0+k][ie] * ring[k][s][g]
-// The k sum runs in the reference's order (ascending j) directly into the
-// accumulator.  Targets are processed in register chunks of MSB; the
-// coefficients (transposed to [j][ie]) are read with uniform addresses,
-// i.e. through the scalar cache.
-template <typename R, int MSB>
-__global__ void __launch_bounds__(256) k_accum(const R *__restrict__ ring, long slot, int cnt,
-                                               const double *__restrict__ anT, long j0, int ms,
-                                               double *__restrict__ acc, long N) {
-  const long s = blockIdx.y;
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= N) return;
-  const R *rp = ring + s * N + g;
-  for (int i0 = 0; i0 < ms; i0 += MSB) {
-    double a[MSB];
-    double *ap = acc + (s * ms + i0) * N + g;
-#pragma unroll
-    for (int q = 0; q < MSB; q++) a[q] = (i0 + q < ms) ? ap[(long)q * N] : 0.0;
-    const double *cp = anT + j0 * ms + i0;
-    int k = 0;
-    for (; k + 8 <= cnt; k += 8) {  // 8 Newton vectors in flight per thread
-      double x[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) x[u] = (double)rp[(long)(k + u) * slot];
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-#pragma unroll
-        for (int q = 0; q < MSB; q++)
-          if (i0 + q < ms) a[q] += cp[(long)(k + u) * ms + q] * x[u];
-    }
-    for (; k < cnt; k++) {
-      const double x = (double)rp[(long)k * slot];
-#pragma unroll
-      for (int q = 0; q < MSB; q++)
-        if (i0 + q < ms) a[q] += cp[(long)k * ms + q] * x;
-    }
-#pragma unroll
-    for (int q = 0; q < MSB; q++)
-      if (i0 + q < ms) ap[(long)q * N] = a[q];
-  }
-}
-
-template <typename R>
-int accum_dev(const R *ring, long slot, int cnt, const double *anT_d, long j0, int ms,
-              double *acc, int ns, long N, hipStream_t st) {
-  dim3 grid((unsigned)((N + 255) / 256), (unsigned)ns);
-  if (ms <= 16)
-    hipLaunchKernelGGL((k_accum<R, 16>), grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms,
-                       acc, N);
-  else
-    hipLaunchKernelGGL((k_accum<R, 32>), grid, dim3(256), 0, st, ring, slot, cnt, anT_d, j0, ms,
-                       acc, N);
-  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
-}
-template int accum_dev<double>(const double *, long, int, const double *, long, int, double *,
-                               int, long, hipStream_t);
-template int accum_dev<float>(const float *, long, int, const double *, long, int, double *, int,
-                              long, hipStream_t);
 
 }  // namespace sep
