@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=24)
+    p.add_argument("--pipeline", action=argparse.BooleanOptionalAction, default=True,
+                   help="time the whole filter-diagonalization pipeline of both precisions")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched uneven-grid measurement (8 dots, 64-160^3)")
     p.add_argument("--no-check", action="store_true",
@@ -280,6 +282,41 @@ def main():
         log(f"fp64 filter: {t64 / args.steps:.3f} s/step, {v64:.0f} H.psi/s; live kernel ms "
             f"A {live64[0]:.4f} B {live64[1]:.4f} C {live64[2]:.4f} accum {live64[3]:.3f}")
 
+    # ------------------------------------- whole pipeline at this workload
+    # after one more filter pass of each precision: ortho (FP64: double-double
+    # Gram; FP32: FP32-MFMA Gram), Rayleigh-Ritz, and sigma (FP64) or the FP64
+    # residual-correction pass (FP32 pipeline); wall times per stage
+    pipe = {}
+    if args.pipeline:
+        for name, prec in (("fp32", sep.SEP_PREC_FP32), ("fp64", sep.SEP_PREC_FP64)):
+            if name == "fp64" and args.no_fp64:
+                continue
+            st = {}
+            t0 = time.perf_counter()
+            ctx.filter_dev(psi0_dev.data_ptr(), ns, an, zn, vmin, dE, prec)
+            torch.cuda.synchronize()
+            st["filter"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            if prec == sep.SEP_PREC_FP32:
+                kept, _ = ctx.ortho_fp32(ns * ms)
+            else:
+                kept, _ = ctx.ortho(ns * ms)
+            st["ortho"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            ev = ctx.rayleigh_ritz(kept)
+            st["rayleigh_ritz"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            if prec == sep.SEP_PREC_FP32:
+                ev, sg = ctx.refine(ev)
+                st["refine_and_sigma"] = time.perf_counter() - t0
+            else:
+                sg = ctx.sigma(kept)
+                st["sigma"] = time.perf_counter() - t0
+            pipe[name] = {"kept": int(kept), "s": {k: round(v, 3) for k, v in st.items()},
+                          "total_s": round(sum(st.values()), 3),
+                          "converged_sigma_lt_1e-4": int((sg < 1e-4).sum())}
+            log(f"{name} pipeline: {pipe[name]}")
+
     # ------------------------------------------------------------- roofline
     # Algorithmic bytes per launch, nvec complex vectors of N points
     # (two real states per vector), c = complex bytes, r = real bytes:
@@ -387,6 +424,7 @@ def main():
         "accum_ms_per_block": {"fp32": round(live32[3], 4),
                                "fp64": round(live64[3], 4) if live64 is not None else None},
         "fp32_pipeline_check": check,
+        "pipeline_128": pipe or None,
         "batched_uneven": batched,
         "hf_adjoint": dict(adj, atoms=d["natom"], states=m),
         "cpu_baseline": cpu,

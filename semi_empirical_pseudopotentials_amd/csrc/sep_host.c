@@ -218,6 +218,11 @@ void sep_la_matmul(long n, const double *A, const double *B, double *C) {
 /* Householder tridiagonalisation: A <- H A H column by column; Q collects the
  * reflections (Q^T A0 Q = T).  Then implicit QL with Wilkinson shifts on T,
  * rotations accumulated into Q. */
+struct sep_rot {
+  long i;
+  double c, s;
+};
+
 int sep_la_symeig(long n, double *A, double *w) {
   if (n == 1) {
     w[0] = A[0];
@@ -228,6 +233,10 @@ int sep_la_symeig(long n, double *A, double *w) {
   double *v = (double *)malloc(sizeof(double) * n);
   double *p = (double *)malloc(sizeof(double) * n);
   double *e = (double *)calloc(n, sizeof(double));
+  double *rot_c = (double *)malloc(sizeof(double) * n);
+  double *rot_s = (double *)malloc(sizeof(double) * n);
+  long nrot = 0, caprot = 4 * n + 16;
+  struct sep_rot *rot = (struct sep_rot *)malloc(sizeof(struct sep_rot) * caprot);
   for (long i = 0; i < n; i++) Q[i * n + i] = 1.0;
   for (long k = 0; k + 2 < n; k++) {
     const long m = n - k - 1; /* active block rows k+1..n-1 */
@@ -263,14 +272,26 @@ int sep_la_symeig(long n, double *A, double *w) {
     }
     A[(k + 1) * n + k] = A[k * n + k + 1] = alpha;
     for (long i = 1; i < m; i++) A[(k + 1 + i) * n + k] = A[k * n + k + 1 + i] = 0.0;
-    /* Q[:, k+1:] <- Q[:, k+1:] (I - 2 v v^T) */
-#pragma omp parallel for schedule(static) if (n > 128)
-    for (long r = 0; r < n; r++) {
-      double *qr = Q + r * n + (k + 1);
-      double s = 0.0;
-      for (long j = 0; j < m; j++) s += qr[j] * v[j];
-      s *= 2.0;
-      for (long j = 0; j < m; j++) qr[j] -= s * v[j];
+    /* Q[:, k+1:] <- Q[:, k+1:] (I - 2 v v^T), Q held transposed (QT[j][r] =
+     * Q[r][j]) so that both this update and the QL rotations stream rows */
+#pragma omp parallel if (n > 128)
+    {
+#pragma omp for schedule(static)
+      for (long r = 0; r < n; r++) p[r] = 0.0;
+#pragma omp for schedule(static)
+      for (long r0 = 0; r0 < n; r0 += 64) {
+        const long r1 = r0 + 64 < n ? r0 + 64 : n;
+        for (long j = 0; j < m; j++) {
+          const double *qj = Q + (k + 1 + j) * n;
+          const double vj = v[j];
+          for (long r = r0; r < r1; r++) p[r] += qj[r] * vj;
+        }
+        for (long j = 0; j < m; j++) {
+          double *qj = Q + (k + 1 + j) * n;
+          const double vj = 2.0 * v[j];
+          for (long r = r0; r < r1; r++) qj[r] -= vj * p[r];
+        }
+      }
     }
   }
   for (long i = 0; i < n; i++) w[i] = A[i * n + i];
@@ -297,6 +318,7 @@ int sep_la_symeig(long n, double *A, double *w) {
       double s = 1.0, c = 1.0, pp = 0.0;
       long i;
       int deflated = 0;
+      long ilast = m; /* rotations rot_c/rot_s[m-1 .. ilast] of this sweep */
       for (i = m - 1; i >= l; i--) {
         double f = s * e[i], b = c * e[i];
         r = hypot(f, g);
@@ -314,11 +336,21 @@ int sep_la_symeig(long n, double *A, double *w) {
         pp = s * r;
         w[i + 1] = g + pp;
         g = c * r - b;
-        for (long k = 0; k < n; k++) {
-          double qa = Q[k * n + i + 1], qb = Q[k * n + i];
-          Q[k * n + i + 1] = s * qb + c * qa;
-          Q[k * n + i] = c * qb - s * qa;
+        rot_c[i] = c;
+        rot_s[i] = s;
+        ilast = i;
+      }
+      /* record the sweep's rotations; they are applied to QT after the QL
+       * iteration, all at once (the QL recurrence does not read QT) */
+      for (long ii = m - 1; ii >= ilast; ii--) {
+        if (nrot == caprot) {
+          caprot *= 2;
+          rot = (struct sep_rot *)realloc(rot, sizeof(struct sep_rot) * caprot);
         }
+        rot[nrot].i = ii;
+        rot[nrot].c = rot_c[ii];
+        rot[nrot].s = rot_s[ii];
+        nrot++;
       }
       if (deflated) continue;
       w[l] -= pp;
@@ -326,6 +358,22 @@ int sep_la_symeig(long n, double *A, double *w) {
       e[m] = 0.0;
     }
   }
+  /* apply the recorded rotations in order to the rows of QT; columns are
+   * independent, so column chunks go to separate threads */
+#pragma omp parallel for schedule(dynamic) if (n > 128)
+  for (long k0 = 0; k0 < n; k0 += 32) {
+    const long k1 = k0 + 32 < n ? k0 + 32 : n;
+    for (long t = 0; t < nrot; t++) {
+      const double cc = rot[t].c, ss = rot[t].s;
+      double *qa_row = Q + (rot[t].i + 1) * n, *qb_row = Q + rot[t].i * n;
+      for (long k = k0; k < k1; k++) {
+        const double qa = qa_row[k], qb = qb_row[k];
+        qa_row[k] = ss * qb + cc * qa;
+        qb_row[k] = cc * qb - ss * qa;
+      }
+    }
+  }
+  free(rot);
   /* sort ascending */
   long *idx = (long *)malloc(sizeof(long) * n);
   for (long i = 0; i < n; i++) idx[i] = i;
@@ -340,7 +388,7 @@ int sep_la_symeig(long n, double *A, double *w) {
   double *ws = (double *)malloc(sizeof(double) * n);
   for (long k = 0; k < n; k++) {
     ws[k] = w[idx[k]];
-    for (long r = 0; r < n; r++) A[r * n + k] = Q[r * n + idx[k]];
+    for (long r = 0; r < n; r++) A[r * n + k] = Q[idx[k] * n + r];
   }
   memcpy(w, ws, sizeof(double) * n);
   free(ws);
@@ -349,6 +397,8 @@ int sep_la_symeig(long n, double *A, double *w) {
   free(v);
   free(p);
   free(e);
+  free(rot_c);
+  free(rot_s);
   return status;
 }
 
