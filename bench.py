@@ -315,7 +315,24 @@ def main():
             pipe[name] = {"kept": int(kept), "s": {k: round(v, 3) for k, v in st.items()},
                           "total_s": round(sum(st.values()), 3),
                           "converged_sigma_lt_1e-4": int((sg < 1e-4).sum())}
+            pipe["_ev_" + name] = (ev, sg)
             log(f"{name} pipeline: {pipe[name]}")
+        if "_ev_fp64" in pipe:
+            # the FP32 pipeline's level nearest to every FP64 level converged to
+            # sigma < 1e-4 within 0.5 eV of the lowest such level
+            e64, s64 = pipe.pop("_ev_fp64")
+            e32, _ = pipe.pop("_ev_fp32")
+            conv = s64 < 1e-4
+            if conv.any():
+                e0 = e64[conv].min()
+                edge = conv & (e64 <= e0 + decks.EDGE_BAND_HA)
+                err = [float(np.abs(e32 - e).min() * decks.HARTREE_EV * 1e3) for e in e64[edge]]
+                pipe["edge_levels_fp32_vs_fp64"] = {"levels": int(edge.sum()),
+                                                    "max_abs_err_meV": round(max(err), 6),
+                                                    "abs_err_meV": [round(x, 6) for x in err]}
+                log(f"128^3 edge levels: {pipe['edge_levels_fp32_vs_fp64']}")
+        else:
+            pipe.pop("_ev_fp32", None)
 
     # ------------------------------------------------------------- roofline
     # Algorithmic bytes per launch, nvec complex vectors of N points
