@@ -240,16 +240,23 @@ int sep_hf_adjoint(sep_ctx *ctx, long npairs, const long *pairs, const double *w
  * The SEP_PREC_FP32 pipeline: sep_filter (complex64) -> sep_ortho_fp32 ->
  * sep_rayleigh_ritz (FP64) -> sep_refine (FP64).  No counterpart in the
  * reference, which is FP64 throughout (filter/ortho.c, Hmat.c, energy.c). */
-/* Smallest singular-value ratio an FP32 Gram matrix resolves. */
-#define SEP_FP32_SVD_FLOOR 1.0e-3
-/* portho() (filter/ortho.c:11-52) with the Gram matrix formed on FP32 MFMA
- * from an FP32 copy of the states and the basis A W S^-1 combined on FP32
- * MFMA, then two FP64 Cholesky-QR passes.  Directions with S_i/S_0 below
- * max(cutoff, SEP_FP32_SVD_FLOOR) are dropped. */
+/* Smallest singular-value ratio sep_ortho_fp32 keeps: its FP64 Gram matrix
+ * carries eigenvalue rounding of ~1e-16 l_0, i.e. singular-value ratios are
+ * resolved down to ~1e-8. */
+#define SEP_FP32_SVD_FLOOR 1.0e-7
+/* portho() (filter/ortho.c:11-52) for the FP32 pipeline: Gram matrix of the
+ * resident states (FP64 MFMA), its eigen-decomposition on the device, basis
+ * A W S^-1 (FP64 MFMA), then two FP64 Cholesky-QR passes.  Directions with
+ * S_i/S_0 below max(cutoff, SEP_FP32_SVD_FLOOR) are dropped.  The states of a
+ * complex64 filter carry a noise floor near 1e-5 of S_0, so the floor drops
+ * only exactly dependent directions. */
 int sep_ortho_fp32(sep_ctx *ctx, long m, double cutoff, double *S, long *kept);
-/* One FP64 residual-correction pass on m resident Ritz pairs with values
- * eval: residuals r_i = H psi_i - e_i psi_i join the basis, Rayleigh-Ritz on
- * span{psi, r} in FP64, and the m pairs of smallest sigma are kept (eval and
+/* Shift (Ha) of the kinetic preconditioner of sep_refine. */
+#define SEP_REFINE_SHIFT 1.0
+/* One FP64 correction pass on m resident Ritz pairs with values eval:
+ * residuals r_i = H psi_i - e_i psi_i, preconditioned t_i = (T + shift)^-1 r_i
+ * (T the kinetic energy, applied in reciprocal space), Rayleigh-Ritz on
+ * span{psi, t} in FP64, and the m pairs of smallest sigma are kept (eval and
  * sige overwritten, ascending in energy).  Returns SEP_ERR_NUMERIC if the
  * enlarged basis holds fewer than m directions. */
 int sep_refine(sep_ctx *ctx, long m, double *eval, double *sige);

@@ -1,6 +1,7 @@
 // sep_api.cpp -- the C ABI of libsep (include/sep.h): context, device memory
 // and the host-side orchestration of the filter-diagonalization pipeline.
 #include <hip/hip_runtime.h>
+#include <rocsolver/rocsolver.h>
 
 #include <cmath>
 #include <cstdarg>
@@ -40,6 +41,15 @@ struct sep_ctx {
   hipEvent_t tev[6] = {};
   double tsum[4] = {0, 0, 0, 0};
   long tcount = 0, tacc_count = 0;
+  // dense eigen/Cholesky solves of the subspace stage (rocSOLVER on st),
+  // created on first use; dws holds their eigenvalues, off-diagonal and info
+  rocblas_handle blas = nullptr;
+  void *dws = nullptr;
+  size_t dws_bytes = 0;
+  // all-zero potential: pass C then returns the inverse transform alone
+  // (the kinetic preconditioner of sep_refine)
+  void *zpot = nullptr;
+  size_t zpot_bytes = 0;
   std::string err;
 };
 
@@ -66,7 +76,10 @@ static int fail(sep_ctx *c, int code, const char *fmt, ...) {
 
 static int ensure(sep_ctx *c, void **p, size_t *cap, size_t bytes) {
   if (*cap >= bytes && *p) return SEP_OK;
-  if (*p) hipFree(*p);
+  if (*p) {  // work queued on the old buffer finishes before it is freed
+    hipStreamSynchronize(c->st);
+    hipFree(*p);
+  }
   *p = nullptr;
   *cap = 0;
   if (bytes == 0) return SEP_OK;
@@ -218,6 +231,9 @@ int sep_close(sep_ctx *c) {
     if (p) hipFree(p);
   for (hipEvent_t e : c->tev)
     if (e) hipEventDestroy(e);
+  if (c->blas) rocblas_destroy_handle(c->blas);
+  if (c->dws) hipFree(c->dws);
+  if (c->zpot) hipFree(c->zpot);
   if (c->st) hipStreamDestroy(c->st);
   delete c;
   return SEP_OK;
@@ -802,8 +818,67 @@ extern "C" int sep_filter(sep_ctx *c, const double *psi0, long ns, const double 
   return SEP_OK;
 }
 
+// ------------------------------------------- dense subspace matrices
+// The subspace stage's n x n problems (n = kept states, up to 2 ns ms) are
+// solved on the device by rocSOLVER in place of the reference's host LAPACK
+// calls.  Matrices are row-major and symmetric on input, so the column-major
+// view rocSOLVER takes is the same matrix.
+static int dense_init(sep_ctx *c, long n) {
+  if (!c->blas) {
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+      c->blas = nullptr;
+      return fail(c, SEP_ERR_HIP, "rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(c->blas, c->st) != rocblas_status_success)
+      return fail(c, SEP_ERR_HIP, "rocblas_set_stream failed");
+  }
+  if (n > 0x7fffffffL) return fail(c, SEP_ERR_ARG, "dense solve of order %ld", n);
+  return ensure(c, &c->dws, &c->dws_bytes, sizeof(double) * (2 * (size_t)n + 1));
+}
+
+// Eigen-decomposition of the symmetric n x n matrix A_d (FP64, divide and
+// conquer): eigenvalues ascending into w (host), eigenvector j left in
+// A_d[j * n .. j * n + n) (column-major).
+static int dev_symeig(sep_ctx *c, long n, double *A_d, double *w, const char *what) {
+  RCHK(dense_init(c, n));
+  double *D = reinterpret_cast<double *>(c->dws), *E = D + n;
+  rocblas_int *info_d = reinterpret_cast<rocblas_int *>(E + n);
+  rocblas_status rs = rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper,
+                                       (rocblas_int)n, A_d, (rocblas_int)n, D, E, info_d);
+  if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "%s: rocsolver_dsyevd status %d", what, (int)rs);
+  rocblas_int info = 0;
+  HIPCHK(c, hipMemcpyAsync(w, D, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (info != 0) return fail(c, SEP_ERR_NUMERIC, "%s: eigensolver did not converge (%d)", what, (int)info);
+  return SEP_OK;
+}
+
+// G_d (symmetric positive definite, row-major) <- R^-1 with R^T R = G, R
+// upper triangular, zeros below the diagonal.  Column-major lower Cholesky
+// L L^T = G is the row-major upper R = L^T; its column-major lower inverse
+// L^-1 is the row-major upper R^-1.
+static int dev_chol_inv(sep_ctx *c, long n, double *G_d, const char *what) {
+  RCHK(dense_init(c, n));
+  rocblas_int *info_d = reinterpret_cast<rocblas_int *>(c->dws);
+  rocblas_int info[2] = {0, 0};
+  rocblas_status rs = rocsolver_dpotrf(c->blas, rocblas_fill_lower, (rocblas_int)n, G_d,
+                                       (rocblas_int)n, info_d);
+  if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "%s: rocsolver_dpotrf status %d", what, (int)rs);
+  HIPCHK(c, hipMemcpyAsync(&info[0], info_d, sizeof(rocblas_int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (info[0] != 0) return fail(c, SEP_ERR_NUMERIC, "%s: Cholesky pivot %d not positive", what, (int)info[0]);
+  rs = rocsolver_dtrtri(c->blas, rocblas_fill_lower, rocblas_diagonal_non_unit, (rocblas_int)n, G_d,
+                        (rocblas_int)n, info_d);
+  if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "%s: rocsolver_dtrtri status %d", what, (int)rs);
+  HIPCHK(c, hipMemcpyAsync(&info[1], info_d, sizeof(rocblas_int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (info[1] != 0) return fail(c, SEP_ERR_NUMERIC, "%s: singular triangular factor (%d)", what, (int)info[1]);
+  return zero_lower_dev(G_d, n, c->st) == SEP_OK ? SEP_OK : fail(c, SEP_ERR_HIP, "%s: zero_lower", what);
+}
+
 // Two Cholesky-QR passes in FP64 on the k resident states: G = A A^T (MFMA),
-// R = chol(G) on the host, A <- R^-T A (MFMA).  Leaves the states
+// R = chol(G) and R^-1 on the device, A <- R^-T A (MFMA).  Leaves the states
 // orthonormal to FP64 rounding.
 static int cholqr2(sep_ctx *c, long k) {
   const long N = c->gd.N;
@@ -813,13 +888,7 @@ static int cholqr2(sep_ctx *c, long k) {
     RCHK(ensure_scr(c, gl + k * k));
     double *G_d = c->scr + gl;
     RCHK(gram_dev<double>(c->states, k, c->states, k, N, G_d, c->scr, gl, c->st));
-    std::vector<double> G(k * k), Ri(k * k);
-    HIPCHK(c, hipMemcpyAsync(G.data(), G_d, sizeof(double) * k * k, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    int cs = sep_la_chol_upper(k, G.data());
-    if (cs) return fail(c, SEP_ERR_NUMERIC, "ortho: re-orthogonalisation Cholesky failed (%d)", cs);
-    sep_la_trinv_upper(k, G.data(), Ri.data());
-    HIPCHK(c, hipMemcpyAsync(G_d, Ri.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, c->st));
+    RCHK(dev_chol_inv(c, k, G_d, "ortho: re-orthogonalisation"));
     RCHK(combine_dev<double>(c->states, k, G_d, k, k, N, c->states2, c->st));
     std::swap(c->states, c->states2);
     std::swap(c->cap1, c->cap2);
@@ -894,52 +963,34 @@ extern "C" int sep_rayleigh_ritz(sep_ctx *c, long m, double *eval) {
   RCHK(ensure_states(c, m, 2));
   RCHK(apply_states(c, c->states, c->states2, m, nullptr, nullptr));  // H psi_i
   const long gl = gram_work_len(m, m, N);
-  RCHK(ensure_scr(c, gl + m * m));
-  double *P_d = c->scr + gl;
+  RCHK(ensure_scr(c, gl + 2 * m * m));
+  double *P_d = c->scr + gl, *M_d = P_d + m * m;
   RCHK(gram_dev<double>(c->states, m, c->states2, m, N, P_d, c->scr, gl, c->st));
-  std::vector<double> P(m * m);
-  HIPCHK(c, hipMemcpyAsync(P.data(), P_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));
-  for (long i = 0; i < m; i++)
-    for (long j = 0; j < i; j++) {
-      double a = 0.5 * (P[i * m + j] + P[j * m + i]) * c->dv;
-      P[i * m + j] = P[j * m + i] = a;
-    }
-  for (long i = 0; i < m; i++) P[i * m + i] *= c->dv;
-  if (sep_la_symeig(m, P.data(), eval)) return fail(c, SEP_ERR_NUMERIC, "Rayleigh-Ritz: QL failed");
-  HIPCHK(c, hipMemcpyAsync(P_d, P.data(), sizeof(double) * m * m, hipMemcpyHostToDevice, c->st));
-  RCHK(combine_dev<double>(c->states, m, P_d, m, m, N, c->states2, c->st));
+  RCHK(symmetrize_dev(P_d, m, c->dv, c->st));  // H_ij = dv <psi_i|H|psi_j>
+  RCHK(dev_symeig(c, m, P_d, eval, "Rayleigh-Ritz"));
+  RCHK(eigvec_cols_dev(P_d, m, m, nullptr, false, M_d, c->st));
+  RCHK(combine_dev<double>(c->states, m, M_d, m, m, N, c->states2, c->st));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
   return normalize_dev(c, c->states, m);  // filter/main.c:122
 }
 
-// Orthonormal basis of the span of the m resident states from their Gram
-// matrix in R (FP32: MFMA with FP32 products; FP64: MFMA FP64): G = W L W^T
-// on the host, directions with sqrt(l_i / l_0) < floor dropped, basis
-// A W_k L_k^-1/2 formed on MFMA in R and left in c->states (FP64), then
-// polished by two FP64 Cholesky-QR passes.  S gets sqrt(l) descending.
-template <typename R>
+// Orthonormal basis of the span of the m resident states from their FP64
+// Gram matrix (MFMA): G = W L W^T on the device, directions with
+// sqrt(l_i / l_0) < floor dropped, basis A W_k L_k^-1/2 formed on MFMA and
+// left in c->states, then polished by two FP64 Cholesky-QR passes.  S gets
+// sqrt(l) descending.
 static int eig_basis(sep_ctx *c, long m, double floor, double *S, long *kept) {
   const long N = c->gd.N;
   RCHK(ensure_states(c, m, 2));
-  R *A = reinterpret_cast<R *>(c->states2);
-  R *U = A + m * N;  // FP32: the second half of states2; FP64: unused
-  if (sizeof(R) == 4) {
-    RCHK(convert_d2f(c->states, reinterpret_cast<float *>(A), m * N, c->st));
-  } else {
-    A = reinterpret_cast<R *>(c->states);
-  }
+  const double *A = c->states;
   const long gl = gram_work_len(m, m, N);
-  RCHK(ensure_scr(c, gl + m * m));
-  double *G_d = c->scr + gl;
-  RCHK(gram_dev<R>(A, m, A, m, N, G_d, c->scr, gl, c->st));
-  std::vector<double> G(m * m), w(m);
-  HIPCHK(c, hipMemcpyAsync(G.data(), G_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));
-  for (long i = 0; i < m; i++)
-    for (long j = 0; j < i; j++) G[i * m + j] = G[j * m + i] = 0.5 * (G[i * m + j] + G[j * m + i]);
-  if (sep_la_symeig(m, G.data(), w.data())) return fail(c, SEP_ERR_NUMERIC, "ortho: eig failed");
+  RCHK(ensure_scr(c, gl + 2 * m * m + m));
+  double *G_d = c->scr + gl, *M_d = G_d + m * m, *sc_d = M_d + m * m;
+  RCHK(gram_dev<double>(A, m, A, m, N, G_d, c->scr, gl, c->st));
+  RCHK(symmetrize_dev(G_d, m, 1.0, c->st));
+  std::vector<double> w(m);
+  RCHK(dev_symeig(c, m, G_d, w.data(), "ortho"));
   std::vector<double> s(m);
   for (long i = 0; i < m; i++) s[i] = sqrt(w[m - 1 - i] > 0.0 ? w[m - 1 - i] : 0.0);
   if (!(s[0] > 0.0)) return fail(c, SEP_ERR_NUMERIC, "ortho: all states are zero");
@@ -950,21 +1001,15 @@ static int eig_basis(sep_ctx *c, long m, double floor, double *S, long *kept) {
       break;
     }
   if (S) memcpy(S, s.data(), sizeof(double) * m);
-  std::vector<double> M(m * k);
-  for (long i = 0; i < m; i++)
-    for (long j = 0; j < k; j++) M[i * k + j] = G[i * m + (m - 1 - j)] / s[j];
-  double *M_d = c->scr + gl;
-  HIPCHK(c, hipMemcpyAsync(M_d, M.data(), sizeof(double) * m * k, hipMemcpyHostToDevice, c->st));
-  if (sizeof(R) == 4) {
-    RCHK(combine_dev<float>(reinterpret_cast<float *>(A), m, M_d, k, k, N,
-                            reinterpret_cast<float *>(U), c->st));
-    RCHK(convert_f2d(reinterpret_cast<float *>(U), c->states, k * N, c->st));
-  } else {
-    RCHK(combine_dev<double>(c->states, m, M_d, k, k, N, c->states2, c->st));
-    std::swap(c->states, c->states2);
-    std::swap(c->cap1, c->cap2);
-  }
-  HIPCHK(c, hipStreamSynchronize(c->st));  // M outlives the async copy
+  // basis j = A v_(m-1-j) / s_j: eigenvectors in descending order, scaled
+  std::vector<double> sc(k);
+  for (long j = 0; j < k; j++) sc[j] = 1.0 / s[j];
+  HIPCHK(c, hipMemcpyAsync(sc_d, sc.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->st));
+  RCHK(eigvec_cols_dev(G_d, m, k, sc_d, true, M_d, c->st));
+  RCHK(combine_dev<double>(c->states, m, M_d, k, k, N, c->states2, c->st));
+  std::swap(c->states, c->states2);
+  std::swap(c->cap1, c->cap2);
+  HIPCHK(c, hipStreamSynchronize(c->st));  // sc outlives the async copy
   RCHK(cholqr2(c, k));
   *kept = k;
   return normalize_dev(c, c->states, k);
@@ -973,11 +1018,47 @@ static int eig_basis(sep_ctx *c, long m, double floor, double *S, long *kept) {
 extern "C" int sep_ortho_fp32(sep_ctx *c, long m, double cutoff, double *S, long *kept) {
   if (!c || m < 1 || m > c->cap1 || !kept || !(cutoff >= 0.0)) return SEP_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
-  return eig_basis<float>(c, m, cutoff > SEP_FP32_SVD_FLOOR ? cutoff : SEP_FP32_SVD_FLOOR, S, kept);
+  return eig_basis(c, m, cutoff > SEP_FP32_SVD_FLOOR ? cutoff : SEP_FP32_SVD_FLOOR, S, kept);
 }
 
-// One FP64 residual-correction pass on m resident Ritz pairs (values e):
-// r_i = H psi_i - e_i psi_i, normalised, Rayleigh-Ritz in span{Psi, R}
+// dst = P src for m real states (FP64), P = 1 / (T + shift) with T the
+// clamped kinetic energy of the H.psi passes: forward FFT, multiply by
+// 1 / (N (T + shift)), inverse FFT.  Pass B takes the preconditioner factor
+// in place of T; a zero potential leaves pass C's epilogue the transform.
+static int precond_states(sep_ctx *c, const double *src, double *dst, long m, double shift) {
+  const long N = c->gd.N;
+  if (!c->zpot) {
+    RCHK(ensure(c, &c->zpot, &c->zpot_bytes, sizeof(double) * N));
+    RCHK(zero_dev(c->zpot, sizeof(double) * N, c->st));
+  }
+  GridDev gd = c->gd;
+  gd.pnum = 1.0 / ((double)N * (double)N);
+  gd.pden = shift / (double)N;
+  const double *zp = reinterpret_cast<const double *>(c->zpot);
+  auto run = [&](VecView<double> in, VecView<double> out, long nv) -> int {
+    RCHK(ensure(c, &c->tb, &c->tb_bytes, sizeof(cx<double>) * (size_t)nv * N));
+    long nblk = 0;
+    int rc = hpsi_apply<double>(gd, in, out, reinterpret_cast<cx<double> *>(c->tb), zp, (int)nv,
+                                1.0, 0.0, nullptr, &nblk, c->st);
+    return rc == SEP_OK ? SEP_OK : fail(c, rc, "preconditioner launch failed");
+  };
+  const long npair = m / 2;
+  for (long p0 = 0; p0 < npair; p0 += 64) {  // two states per complex vector
+    const long np = npair - p0 < 64 ? npair - p0 : 64;
+    double *s0 = const_cast<double *>(src) + 2 * p0 * N, *d0 = dst + 2 * p0 * N;
+    RCHK(run(VecView<double>{s0, s0 + N, nullptr, 2 * N}, VecView<double>{d0, d0 + N, nullptr, 2 * N},
+             np));
+  }
+  if (m & 1) {
+    double *s0 = const_cast<double *>(src) + (m - 1) * N, *d0 = dst + (m - 1) * N;
+    RCHK(run(VecView<double>{s0, nullptr, nullptr, N}, VecView<double>{d0, nullptr, nullptr, N}, 1));
+  }
+  return SEP_OK;
+}
+
+// One FP64 correction pass on m resident Ritz pairs (values e): residuals
+// r_i = H psi_i - e_i psi_i, preconditioned t_i = P r_i (P as in
+// precond_states, shift SEP_REFINE_SHIFT), Rayleigh-Ritz in span{Psi, T}
 // (orthonormalised from the FP64 Gram with singular-value floor 1e-7), and
 // the m Ritz pairs of smallest energy variance kept, ascending in energy.
 extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
@@ -994,9 +1075,15 @@ extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
   RCHK(sub_scaled_rows_dev(c->states2 + m * N, N, c->states, N, m, N, c->scr, c->st));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
+  // t_i = P r_i into the second block of states2, copied back over r_i
+  // (states2 is the former states buffer: grow it to 2m rows first)
+  RCHK(ensure_states(c, 2 * m, 2));
+  RCHK(precond_states(c, c->states + m * N, c->states2 + m * N, m, SEP_REFINE_SHIFT));
+  HIPCHK(c, hipMemcpyAsync(c->states + m * N, c->states2 + m * N, sizeof(double) * m * N,
+                           hipMemcpyDeviceToDevice, c->st));
   RCHK(normalize_dev(c, c->states + m * N, m));
   long k = 0;
-  RCHK(eig_basis<double>(c, 2 * m, 1.0e-7, nullptr, &k));
+  RCHK(eig_basis(c, 2 * m, SEP_FP32_SVD_FLOOR, nullptr, &k));
   std::vector<double> ev(k), sg(k);
   RCHK(sep_rayleigh_ritz(c, k, ev.data()));
   RCHK(sep_sigma(c, k, sg.data()));

@@ -191,13 +191,15 @@ __global__ void __launch_bounds__(W * FftPlan<NL>::G) k_lines(cx<R> *tb, long N,
 }
 
 // ======================================== z pencils: forward, * k^2, backward
-template <typename R, int NZ, int W>
+// PRE: the factor is kscale / (kin + pden) (kinetic preconditioner) instead
+// of kin * kscale
+template <typename R, int NZ, int W, bool PRE = false>
 __global__ void __launch_bounds__(W * FftPlan<NZ>::G) k_pencil(cx<R> *tb, long N, int nx, int ny,
                                                                const cx<R> *twz,
                                                                const double *kxx,
                                                                const double *kyy,
                                                                const double *kzz, double ekin,
-                                                               double kscale) {
+                                                               double kscale, double pden) {
   using PL = FftPlan<NZ>;
   constexpr int G = PL::G, E = PL::E, P = PL::P;
   __shared__ cx<R> buf[NZ][W];
@@ -234,7 +236,10 @@ __global__ void __launch_bounds__(W * FftPlan<NZ>::G) k_pencil(cx<R> *tb, long N
       int z = out_idx<NZ, -1, P - 1>(g, b, k);
       double kin = kxy + kz_s[z];
       if (kin > ekin) kin = ekin;
-      v[b * rl_ + k] = cscale(v[b * rl_ + k], (R)(kin * kscale));
+      if constexpr (PRE)
+        v[b * rl_ + k] = cscale(v[b * rl_ + k], (R)(kscale / (kin + pden)));
+      else
+        v[b * rl_ + k] = cscale(v[b * rl_ + k], (R)(kin * kscale));
     }
   // backward: pass 0 reads exactly the registers the forward last pass wrote
   run_passes<NZ, 1, 0, P>(v, g, act, tw, at);
@@ -720,7 +725,7 @@ struct HOps {
                     int nvec, const cx<R> *tw, hipStream_t st);
   static bool pencil(int nz, cx<R> *tb, long N, int nx, int ny, int nvec, const cx<R> *tw,
                      const double *kxx, const double *kyy, const double *kzz, double ekin,
-                     double kscale, hipStream_t st);
+                     double kscale, double pden, hipStream_t st);
   static bool plane_fwd(int nxy, VecView<R> in, cx<R> *tb, long N, int nz, int nvec,
                         const cx<R> *tw, hipStream_t st);
   static bool plane_bwd(int nxy, const cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
@@ -787,17 +792,26 @@ bool HOps<R>::lines(int nl, int S, cx<R> *tb, long N, int nx, long es, long os, 
   return false;
 }
 
+// pden > 0: preconditioner factor kscale / (kin + pden), FP64 only
 template <typename R>
 bool HOps<R>::pencil(int nz, cx<R> *tb, long N, int nx, int ny, int nvec, const cx<R> *tw,
                      const double *kxx, const double *kyy, const double *kzz, double ekin,
-                     double kscale, hipStream_t st) {
+                     double kscale, double pden, hipStream_t st) {
   constexpr int W = sizeof(R) == 8 ? 16 : 32;
   dim3 grid((unsigned)((nx + W - 1) / W), (unsigned)ny, (unsigned)nvec);
+  if (pden > 0.0 && sizeof(R) != 8) return false;
   switch (nz) {
 #define C(n)                                                                                  \
   case n:                                                                                     \
+    if constexpr (sizeof(R) == 8) {                                                           \
+      if (pden > 0.0) {                                                                       \
+        hipLaunchKernelGGL((k_pencil<R, n, W, true>), grid, dim3(W * FftPlan<n>::G), 0, st, tb, \
+                           N, nx, ny, tw, kxx, kyy, kzz, ekin, kscale, pden);                 \
+        return true;                                                                          \
+      }                                                                                       \
+    }                                                                                         \
     hipLaunchKernelGGL((k_pencil<R, n, W>), grid, dim3(W * FftPlan<n>::G), 0, st, tb, N, nx, ny, \
-                       tw, kxx, kyy, kzz, ekin, kscale);                                      \
+                       tw, kxx, kyy, kzz, ekin, kscale, 0.0);                                 \
     return true;
     SEP_LINE_SIZES(C)
 #undef C
@@ -993,7 +1007,8 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
   // pass B (a column-major plane spectrum swaps the roles of x and y)
   const bool cm = plane && plane_is_colmajor(nx);
   if (!HOps<R>::pencil(nz, tb, N, cm ? ny : nx, cm ? nx : ny, nvec, twz, cm ? gd.kyy : gd.kxx,
-                       cm ? gd.kxx : gd.kyy, gd.kzz, gd.ekin, 1.0, st))
+                       cm ? gd.kxx : gd.kyy, gd.kzz, gd.ekin, gd.pden > 0.0 ? gd.pnum : 1.0,
+                       gd.pden, st))
     return SEP_ERR_UNSUPPORTED;
   if (ev) hipEventRecord(ev[2], st);
   // pass C

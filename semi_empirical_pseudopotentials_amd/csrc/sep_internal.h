@@ -16,6 +16,9 @@ struct GridDev {
   void *tw32[3];  // same, complex64
   double *kxx, *kyy, *kzz;  // filter/init.c:125-135 axis tables (1/N folded in)
   double ekin;              // Ekinmax / N (filter/init.c:137)
+  // pden > 0 turns pass B into the kinetic preconditioner: the spectrum is
+  // multiplied by pnum / (kin + pden) instead of kin (kin with 1/N folded in)
+  double pnum, pden;
   // per precision (SEP_PREC_FP64 = 0, SEP_PREC_FP32 = 1): use the row/line
   // kernels even where a plane kernel exists (sep_autotune, or the test hook
   // SEP_FORCE_LINE_KERNELS)
@@ -61,6 +64,11 @@ int sub_scaled_rows_dev(double *y, long ldy, const double *x, long ldx, long m, 
 int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStream_t st);
 int convert_d2f(const double *in, float *out, long n, hipStream_t st);
 int convert_f2d(const float *in, double *out, long n, hipStream_t st);
+// subspace-stage helpers on n x n row-major matrices (sep_kernels.hip)
+int symmetrize_dev(double *A, long n, double scale, hipStream_t st);
+int eigvec_cols_dev(const double *V, long n, long k, const double *sc_d, bool reverse, double *M,
+                    hipStream_t st);
+int zero_lower_dev(double *A, long n, hipStream_t st);
 
 // Newton accumulation on the matrix cores (sep_gemm.hip):
 // acc[(s*ms + ie)*N + g] += sum_{k<cnt} anT[(j0 + k)*ms + ie] * ring row (s, k)

@@ -175,4 +175,62 @@ int convert_f2d(const float *in, double *out, long n, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 
+// ---- small dense matrices of the subspace stage (n <= a few thousand)
+
+// A <- scale * (A + A^T) / 2 for an n x n row-major matrix; the (i > j)
+// thread of each pair writes both entries, the diagonal is only scaled
+__global__ void k_symmetrize(double *A, long n, double scale) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long i = (long)blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= n || j >= n || j > i) return;
+  if (i == j) {
+    A[i * n + i] *= scale;
+    return;
+  }
+  const double a = 0.5 * (A[i * n + j] + A[j * n + i]) * scale;
+  A[i * n + j] = a;
+  A[j * n + i] = a;
+}
+int symmetrize_dev(double *A, long n, double scale, hipStream_t st) {
+  const dim3 blk(32, 8), grd((unsigned)((n + 31) / 32), (unsigned)((n + 7) / 8));
+  hipLaunchKernelGGL(k_symmetrize, grd, blk, 0, st, A, n, scale);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
+// M[i * k + j] = V[c(j) * n + i] * (sc ? sc[j] : 1), c(j) = j or n - 1 - j:
+// eigenvector columns (column-major V, as the eigensolver leaves them) to the
+// row-major coefficient matrix of combine_dev, through a 32 x 32 LDS tile
+__global__ void k_eigvec_cols(const double *V, long n, long k, const double *sc, int reverse,
+                              double *M) {
+  __shared__ double t[32][33];
+  const long i0 = (long)blockIdx.x * 32, j0 = (long)blockIdx.y * 32;
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {  // r: output column j0 + r
+    const long j = j0 + r, i = i0 + threadIdx.x;
+    if (j < k && i < n) t[r][threadIdx.x] = V[(reverse ? n - 1 - j : j) * n + i];
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {  // r: row i0 + r
+    const long i = i0 + r, j = j0 + threadIdx.x;
+    if (i < n && j < k) M[i * k + j] = t[threadIdx.x][r] * (sc ? sc[j] : 1.0);
+  }
+}
+int eigvec_cols_dev(const double *V, long n, long k, const double *sc_d, bool reverse, double *M,
+                    hipStream_t st) {
+  const dim3 blk(32, 8), grd((unsigned)((n + 31) / 32), (unsigned)((k + 31) / 32));
+  hipLaunchKernelGGL(k_eigvec_cols, grd, blk, 0, st, V, n, k, sc_d, reverse ? 1 : 0, M);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
+// zero the strictly lower triangle (i > j) of an n x n row-major matrix
+__global__ void k_zero_lower(double *A, long n) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long i = (long)blockIdx.y * blockDim.y + threadIdx.y;
+  if (i < n && j < i) A[i * n + j] = 0.0;
+}
+int zero_lower_dev(double *A, long n, hipStream_t st) {
+  const dim3 blk(32, 8), grd((unsigned)((n + 31) / 32), (unsigned)((n + 7) / 8));
+  hipLaunchKernelGGL(k_zero_lower, grd, blk, 0, st, A, n);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
 }  // namespace sep
