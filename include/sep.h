@@ -134,7 +134,7 @@ int sep_filter_dev(sep_ctx *ctx, const void *psi0_dev, long ns, const double *an
 /* ------------------------------------------------- dense device contractions */
 /* On device buffers of the context's grid (N = nx ny nz points per state,
  * state-major; FP64 when precision = SEP_PREC_FP64, FP32 otherwise), on the
- * matrix cores:
+ * matrix cores (FP64: rocBLAS dgemm; FP32: this library's MFMA kernels):
  *   sep_gram_dev    C[i][j] = sum_g A[i][g] B[j][g]   (C: host, m x mb, FP64)
  *   sep_rotate_dev  Out[j][g] = sum_i M[i][j] A[i][g] (M: host, m x mo row-major)
  * They are the projected-H / overlap contractions and the Ritz rotation of
@@ -245,8 +245,8 @@ int sep_hf_adjoint(sep_ctx *ctx, long npairs, const long *pairs, const double *w
  * resolved down to ~1e-8. */
 #define SEP_FP32_SVD_FLOOR 1.0e-7
 /* portho() (filter/ortho.c:11-52) for the FP32 pipeline: Gram matrix of the
- * resident states (FP64 MFMA), its eigen-decomposition on the device, basis
- * A W S^-1 (FP64 MFMA), then two FP64 Cholesky-QR passes.  Directions with
+ * resident states (FP64 dgemm), its eigen-decomposition on the device, basis
+ * A W S^-1 (FP64 dgemm), then two FP64 Cholesky-QR passes.  Directions with
  * S_i/S_0 below max(cutoff, SEP_FP32_SVD_FLOOR) are dropped.  The states of a
  * complex64 filter carry a noise floor near 1e-5 of S_0, so the floor drops
  * only exactly dependent directions. */

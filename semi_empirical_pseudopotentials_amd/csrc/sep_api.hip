@@ -829,8 +829,9 @@ static int dense_init(sep_ctx *c, long n) {
       c->blas = nullptr;
       return fail(c, SEP_ERR_HIP, "rocblas_create_handle failed");
     }
-    if (rocblas_set_stream(c->blas, c->st) != rocblas_status_success)
-      return fail(c, SEP_ERR_HIP, "rocblas_set_stream failed");
+    if (rocblas_set_stream(c->blas, c->st) != rocblas_status_success ||
+        rocblas_set_atomics_mode(c->blas, rocblas_atomics_not_allowed) != rocblas_status_success)
+      return fail(c, SEP_ERR_HIP, "rocblas_set_stream / atomics mode failed");
   }
   if (n > 0x7fffffffL) return fail(c, SEP_ERR_ARG, "dense solve of order %ld", n);
   return ensure(c, &c->dws, &c->dws_bytes, sizeof(double) * (2 * (size_t)n + 1));
@@ -877,19 +878,54 @@ static int dev_chol_inv(sep_ctx *c, long n, double *G_d, const char *what) {
   return zero_lower_dev(G_d, n, c->st) == SEP_OK ? SEP_OK : fail(c, SEP_ERR_HIP, "%s: zero_lower", what);
 }
 
-// Two Cholesky-QR passes in FP64 on the k resident states: G = A A^T (MFMA),
-// R = chol(G) and R^-1 on the device, A <- R^-T A (MFMA).  Leaves the states
+// C[i][j] = sum_g A[i][g] B[j][g] (m x mb, row-major, device) for state-major
+// FP64 A (m x N) and B (mb x N): rocBLAS dgemm, no atomics (deterministic),
+// in blocks of at most 512 x 512 -- at K = N ~ 1e6 its kernel choice for a
+// single 1024 x 1024 product ran at a fifth of the 512-block rate (13 vs 60
+// TFLOP/s measured at N = 128^3).  Column-major X (mb x m, ld mb) with
+// X(j, i) = C[i][j] is B^T A.
+static int sub_gram(sep_ctx *c, const double *A, long m, const double *B, long mb, double *C_d) {
+  RCHK(dense_init(c, 0));
+  const long N = c->gd.N;
+  const double one = 1.0, zero = 0.0;
+  for (long i0 = 0; i0 < m; i0 += 512)
+    for (long j0 = 0; j0 < mb; j0 += 512) {
+      const long mi = std::min(512L, m - i0), mj = std::min(512L, mb - j0);
+      rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none,
+                                        (rocblas_int)mj, (rocblas_int)mi, (rocblas_int)N, &one,
+                                        B + j0 * N, (rocblas_int)N, A + i0 * N, (rocblas_int)N, &zero,
+                                        C_d + j0 + i0 * mb, (rocblas_int)mb);
+      if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "Gram: rocblas_dgemm status %d", (int)rs);
+    }
+  return SEP_OK;
+}
+
+// Out[j][g] = sum_i M[i * ldm + j] A[i][g] (mo states out, Out != A):
+// column-major Out (N x mo) = A (N x m) M^T with M column-major (ldm x m).
+static int sub_combine(sep_ctx *c, const double *A, long m, const double *M_d, long ldm, long mo,
+                       double *Out) {
+  RCHK(dense_init(c, 0));
+  const long N = c->gd.N;
+  const double one = 1.0, zero = 0.0;
+  rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose,
+                                    (rocblas_int)N, (rocblas_int)mo, (rocblas_int)m, &one, A,
+                                    (rocblas_int)N, M_d, (rocblas_int)ldm, &zero, Out, (rocblas_int)N);
+  return rs == rocblas_status_success ? SEP_OK
+                                      : fail(c, SEP_ERR_HIP, "rotation: rocblas_dgemm status %d", (int)rs);
+}
+
+// Two Cholesky-QR passes in FP64 on the k resident states: G = A A^T,
+// R = chol(G) and R^-1, A <- R^-T A, all on the device.  Leaves the states
 // orthonormal to FP64 rounding.
 static int cholqr2(sep_ctx *c, long k) {
   const long N = c->gd.N;
   RCHK(ensure_states(c, k, 2));
   for (int pass = 0; pass < 2; pass++) {
-    const long gl = gram_work_len(k, k, N);
-    RCHK(ensure_scr(c, gl + k * k));
-    double *G_d = c->scr + gl;
-    RCHK(gram_dev<double>(c->states, k, c->states, k, N, G_d, c->scr, gl, c->st));
+    RCHK(ensure_scr(c, k * k));
+    double *G_d = c->scr;
+    RCHK(sub_gram(c, c->states, k, c->states, k, G_d));
     RCHK(dev_chol_inv(c, k, G_d, "ortho: re-orthogonalisation"));
-    RCHK(combine_dev<double>(c->states, k, G_d, k, k, N, c->states2, c->st));
+    RCHK(sub_combine(c, c->states, k, G_d, k, k, c->states2));
     std::swap(c->states, c->states2);
     std::swap(c->cap1, c->cap2);
   }
@@ -947,7 +983,7 @@ extern "C" int sep_ortho(sep_ctx *c, long m, double cutoff, double *S, long *kep
   RCHK(ensure_scr(c, wl + 2 * m * m));
   double *M_d = c->scr + wl;
   HIPCHK(c, hipMemcpyAsync(M_d, M.data(), sizeof(double) * m * k, hipMemcpyHostToDevice, c->st));
-  RCHK(combine_dev<double>(c->states, m, M_d, k, k, N, c->states2, c->st));
+  RCHK(sub_combine(c, c->states, m, M_d, k, k, c->states2));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
   RCHK(cholqr2(c, k));
@@ -962,32 +998,29 @@ extern "C" int sep_rayleigh_ritz(sep_ctx *c, long m, double *eval) {
   const long N = c->gd.N;
   RCHK(ensure_states(c, m, 2));
   RCHK(apply_states(c, c->states, c->states2, m, nullptr, nullptr));  // H psi_i
-  const long gl = gram_work_len(m, m, N);
-  RCHK(ensure_scr(c, gl + 2 * m * m));
-  double *P_d = c->scr + gl, *M_d = P_d + m * m;
-  RCHK(gram_dev<double>(c->states, m, c->states2, m, N, P_d, c->scr, gl, c->st));
+  RCHK(ensure_scr(c, 2 * m * m));
+  double *P_d = c->scr, *M_d = P_d + m * m;
+  RCHK(sub_gram(c, c->states, m, c->states2, m, P_d));
   RCHK(symmetrize_dev(P_d, m, c->dv, c->st));  // H_ij = dv <psi_i|H|psi_j>
   RCHK(dev_symeig(c, m, P_d, eval, "Rayleigh-Ritz"));
   RCHK(eigvec_cols_dev(P_d, m, m, nullptr, false, M_d, c->st));
-  RCHK(combine_dev<double>(c->states, m, M_d, m, m, N, c->states2, c->st));
+  RCHK(sub_combine(c, c->states, m, M_d, m, m, c->states2));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
   return normalize_dev(c, c->states, m);  // filter/main.c:122
 }
 
 // Orthonormal basis of the span of the m resident states from their FP64
-// Gram matrix (MFMA): G = W L W^T on the device, directions with
-// sqrt(l_i / l_0) < floor dropped, basis A W_k L_k^-1/2 formed on MFMA and
-// left in c->states, then polished by two FP64 Cholesky-QR passes.  S gets
+// Gram matrix: G = W L W^T on the device, directions with sqrt(l_i / l_0) <
+// floor dropped, basis A W_k L_k^-1/2 left in c->states, then polished by two FP64 Cholesky-QR passes.  S gets
 // sqrt(l) descending.
 static int eig_basis(sep_ctx *c, long m, double floor, double *S, long *kept) {
   const long N = c->gd.N;
   RCHK(ensure_states(c, m, 2));
   const double *A = c->states;
-  const long gl = gram_work_len(m, m, N);
-  RCHK(ensure_scr(c, gl + 2 * m * m + m));
-  double *G_d = c->scr + gl, *M_d = G_d + m * m, *sc_d = M_d + m * m;
-  RCHK(gram_dev<double>(A, m, A, m, N, G_d, c->scr, gl, c->st));
+  RCHK(ensure_scr(c, 2 * m * m + m));
+  double *G_d = c->scr, *M_d = G_d + m * m, *sc_d = M_d + m * m;
+  RCHK(sub_gram(c, A, m, A, m, G_d));
   RCHK(symmetrize_dev(G_d, m, 1.0, c->st));
   std::vector<double> w(m);
   RCHK(dev_symeig(c, m, G_d, w.data(), "ortho"));
@@ -1006,7 +1039,7 @@ static int eig_basis(sep_ctx *c, long m, double floor, double *S, long *kept) {
   for (long j = 0; j < k; j++) sc[j] = 1.0 / s[j];
   HIPCHK(c, hipMemcpyAsync(sc_d, sc.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->st));
   RCHK(eigvec_cols_dev(G_d, m, k, sc_d, true, M_d, c->st));
-  RCHK(combine_dev<double>(c->states, m, M_d, k, k, N, c->states2, c->st));
+  RCHK(sub_combine(c, c->states, m, M_d, k, k, c->states2));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
   HIPCHK(c, hipStreamSynchronize(c->st));  // sc outlives the async copy
@@ -1101,11 +1134,10 @@ extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
     sige[j] = sg[idx[j]];
   }
   for (long j = keep; j < m; j++) eval[j] = sige[j] = 0.0;
-  const long gl = gram_work_len(k, keep, N);
-  RCHK(ensure_scr(c, gl + k * keep));
+  RCHK(ensure_scr(c, k * keep));
   HIPCHK(c, hipMemcpyAsync(c->scr, P.data(), sizeof(double) * k * keep, hipMemcpyHostToDevice, c->st));
   RCHK(ensure_states(c, keep, 2));
-  RCHK(combine_dev<double>(c->states, k, c->scr, keep, keep, N, c->states2, c->st));
+  RCHK(sub_combine(c, c->states, k, c->scr, keep, keep, c->states2));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
   HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1339,7 +1371,7 @@ extern "C" int sep_gram_dev(sep_ctx *c, int precision, const void *A, long m, co
   RCHK(ensure_scr(c, gl + m * mb));
   double *C_d = c->scr + gl;
   if (precision == SEP_PREC_FP64)
-    RCHK(gram_dev<double>((const double *)A, m, (const double *)B, mb, N, C_d, c->scr, gl, c->st));
+    RCHK(sub_gram(c, (const double *)A, m, (const double *)B, mb, C_d));
   else
     RCHK(gram_dev<float>((const float *)A, m, (const float *)B, mb, N, C_d, c->scr, gl, c->st));
   HIPCHK(c, hipMemcpyAsync(C, C_d, sizeof(double) * m * mb, hipMemcpyDeviceToHost, c->st));
@@ -1354,7 +1386,7 @@ extern "C" int sep_rotate_dev(sep_ctx *c, int precision, const void *A, long m, 
   RCHK(ensure_scr(c, m * mo));
   HIPCHK(c, hipMemcpyAsync(c->scr, M, sizeof(double) * m * mo, hipMemcpyHostToDevice, c->st));
   if (precision == SEP_PREC_FP64)
-    RCHK(combine_dev<double>((const double *)A, m, c->scr, mo, mo, c->gd.N, (double *)Out, c->st));
+    RCHK(sub_combine(c, (const double *)A, m, c->scr, mo, mo, (double *)Out));
   else
     RCHK(combine_dev<float>((const float *)A, m, c->scr, mo, mo, c->gd.N, (float *)Out, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));

@@ -1,9 +1,12 @@
-// sep_gemm.hip -- matrix-core (MFMA) contractions of the subspace stage:
+// sep_gemm.hip -- matrix-core (MFMA) contractions:
+//   Newton accumulation  acc[s][ie] += sum_k a[ie][k] psi_k[s]  (filter/filter.c:74-78)
 //   Gram / projected H   C[i][j] = sum_g A[i][g] B[j][g]     (filter/Hmat.c:30-46,
 //                                                           the SVD of ortho.c:32)
 //   rotation             Out[j][g] = sum_i M[i][j] A[i][g]   (filter/Hmat.c:61-78)
 // plus the double-double Gram that decides the SVD cutoff in FP64 exactly
-// enough to resolve S_i/S_0 = 1e-10 (filter/ortho.c:41-46).
+// enough to resolve S_i/S_0 = 1e-10 (filter/ortho.c:41-46).  The Gram and
+// rotation kernels serve FP32 operands; the FP64 subspace stage calls
+// rocBLAS dgemm for those two plain products.
 //
 // MFMA 16x16x4 operand layout (wave64, lane l):  A[i = l%16][k = l/16],
 // B[k = l/16][j = l%16]; accumulator element r < 4 of lane l holds
@@ -134,8 +137,6 @@ int gram_dev(const R *A, long m, const R *B, long mb, long N, double *C_d, doubl
   hipLaunchKernelGGL(k_sum_chunks, dim3(nb), dim3(256), 0, st, work, nch, len, C_d);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
-template int gram_dev<double>(const double *, long, const double *, long, long, double *, double *,
-                              long, hipStream_t);
 template int gram_dev<float>(const float *, long, const float *, long, long, double *, double *,
                              long, hipStream_t);
 
@@ -212,8 +213,6 @@ int combine_dev(const R *A, long m, const double *M_d, long ldm, long mo, long N
   hipLaunchKernelGGL(k_combine<R>, grid, dim3(256), 0, st, A, m, M_d, ldm, mo, N, Out);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
-template int combine_dev<double>(const double *, long, const double *, long, long, long, double *,
-                                 hipStream_t);
 template int combine_dev<float>(const float *, long, const double *, long, long, long, float *,
                                 hipStream_t);
 
