@@ -170,238 +170,6 @@ int sep_newton_coeffs(long nc, long ms, const double *zn, const double *el, doub
 }
 
 /* =========================================================== dense FP64 */
-int sep_la_chol_upper(long n, double *A) {
-  for (long j = 0; j < n; j++) {
-    double d = A[j * n + j];
-    for (long k = 0; k < j; k++) d -= A[k * n + j] * A[k * n + j];
-    if (!(d > 0.0)) return (int)(j + 1);
-    double rjj = sqrt(d);
-    A[j * n + j] = rjj;
-#pragma omp parallel for schedule(static) if (n - j > 256)
-    for (long c = j + 1; c < n; c++) {
-      double s = A[j * n + c];
-      for (long k = 0; k < j; k++) s -= A[k * n + j] * A[k * n + c];
-      A[j * n + c] = s / rjj;
-    }
-  }
-  for (long i = 0; i < n; i++)
-    for (long j = 0; j < i; j++) A[i * n + j] = 0.0;
-  return 0;
-}
-
-void sep_la_trinv_upper(long n, const double *R, double *X) {
-  memset(X, 0, sizeof(double) * n * n);
-#pragma omp parallel for schedule(dynamic, 4)
-  for (long c = 0; c < n; c++) { /* solve R x = e_c, back substitution */
-    for (long i = c; i >= 0; i--) {
-      double s = (i == c) ? 1.0 : 0.0;
-      for (long k = i + 1; k <= c; k++) s -= R[i * n + k] * X[k * n + c];
-      X[i * n + c] = s / R[i * n + i];
-    }
-  }
-}
-
-void sep_la_matmul(long n, const double *A, const double *B, double *C) {
-#pragma omp parallel for schedule(static)
-  for (long i = 0; i < n; i++) {
-    double *c = C + i * n;
-    for (long j = 0; j < n; j++) c[j] = 0.0;
-    for (long k = 0; k < n; k++) {
-      double a = A[i * n + k];
-      if (a == 0.0) continue;
-      const double *b = B + k * n;
-      for (long j = 0; j < n; j++) c[j] += a * b[j];
-    }
-  }
-}
-
-/* Householder tridiagonalisation: A <- H A H column by column; Q collects the
- * reflections (Q^T A0 Q = T).  Then implicit QL with Wilkinson shifts on T,
- * rotations accumulated into Q. */
-struct sep_rot {
-  long i;
-  double c, s;
-};
-
-int sep_la_symeig(long n, double *A, double *w) {
-  if (n == 1) {
-    w[0] = A[0];
-    A[0] = 1.0;
-    return 0;
-  }
-  double *Q = (double *)calloc(n * n, sizeof(double));
-  double *v = (double *)malloc(sizeof(double) * n);
-  double *p = (double *)malloc(sizeof(double) * n);
-  double *e = (double *)calloc(n, sizeof(double));
-  double *rot_c = (double *)malloc(sizeof(double) * n);
-  double *rot_s = (double *)malloc(sizeof(double) * n);
-  long nrot = 0, caprot = 4 * n + 16;
-  struct sep_rot *rot = (struct sep_rot *)malloc(sizeof(struct sep_rot) * caprot);
-  for (long i = 0; i < n; i++) Q[i * n + i] = 1.0;
-  for (long k = 0; k + 2 < n; k++) {
-    const long m = n - k - 1; /* active block rows k+1..n-1 */
-    double xn = 0.0;
-    for (long i = 0; i < m; i++) {
-      v[i] = A[(k + 1 + i) * n + k];
-      xn += v[i] * v[i];
-    }
-    xn = sqrt(xn);
-    if (xn == 0.0) continue;
-    double alpha = v[0] > 0.0 ? -xn : xn;
-    v[0] -= alpha;
-    double vn = 0.0;
-    for (long i = 0; i < m; i++) vn += v[i] * v[i];
-    vn = sqrt(vn);
-    if (vn == 0.0) continue;
-    for (long i = 0; i < m; i++) v[i] /= vn;
-    /* p = A_sub v */
-#pragma omp parallel for schedule(static) if (m > 128)
-    for (long i = 0; i < m; i++) {
-      const double *row = A + (k + 1 + i) * n + (k + 1);
-      double s = 0.0;
-      for (long j = 0; j < m; j++) s += row[j] * v[j];
-      p[i] = s;
-    }
-    double K = 0.0;
-    for (long i = 0; i < m; i++) K += v[i] * p[i];
-    for (long i = 0; i < m; i++) p[i] -= K * v[i]; /* q */
-#pragma omp parallel for schedule(static) if (m > 128)
-    for (long i = 0; i < m; i++) {
-      double *row = A + (k + 1 + i) * n + (k + 1);
-      for (long j = 0; j < m; j++) row[j] -= 2.0 * (v[i] * p[j] + p[i] * v[j]);
-    }
-    A[(k + 1) * n + k] = A[k * n + k + 1] = alpha;
-    for (long i = 1; i < m; i++) A[(k + 1 + i) * n + k] = A[k * n + k + 1 + i] = 0.0;
-    /* Q[:, k+1:] <- Q[:, k+1:] (I - 2 v v^T), Q held transposed (QT[j][r] =
-     * Q[r][j]) so that both this update and the QL rotations stream rows */
-#pragma omp parallel if (n > 128)
-    {
-#pragma omp for schedule(static)
-      for (long r = 0; r < n; r++) p[r] = 0.0;
-#pragma omp for schedule(static)
-      for (long r0 = 0; r0 < n; r0 += 64) {
-        const long r1 = r0 + 64 < n ? r0 + 64 : n;
-        for (long j = 0; j < m; j++) {
-          const double *qj = Q + (k + 1 + j) * n;
-          const double vj = v[j];
-          for (long r = r0; r < r1; r++) p[r] += qj[r] * vj;
-        }
-        for (long j = 0; j < m; j++) {
-          double *qj = Q + (k + 1 + j) * n;
-          const double vj = 2.0 * v[j];
-          for (long r = r0; r < r1; r++) qj[r] -= vj * p[r];
-        }
-      }
-    }
-  }
-  for (long i = 0; i < n; i++) w[i] = A[i * n + i];
-  for (long i = 0; i + 1 < n; i++) e[i] = A[i * n + i + 1];
-  e[n - 1] = 0.0;
-  /* implicit QL */
-  int status = 0;
-  for (long l = 0; l < n; l++) {
-    int it = 0;
-    long m;
-    for (;;) {
-      for (m = l; m + 1 < n; m++) {
-        double dd = fabs(w[m]) + fabs(w[m + 1]);
-        if (fabs(e[m]) + dd == dd) break;
-      }
-      if (m == l) break;
-      if (++it > 200) {
-        status = -1;
-        break;
-      }
-      double g = (w[l + 1] - w[l]) / (2.0 * e[l]);
-      double r = hypot(g, 1.0);
-      g = w[m] - w[l] + e[l] / (g + (g >= 0.0 ? r : -r));
-      double s = 1.0, c = 1.0, pp = 0.0;
-      long i;
-      int deflated = 0;
-      long ilast = m; /* rotations rot_c/rot_s[m-1 .. ilast] of this sweep */
-      for (i = m - 1; i >= l; i--) {
-        double f = s * e[i], b = c * e[i];
-        r = hypot(f, g);
-        e[i + 1] = r;
-        if (r == 0.0) {
-          w[i + 1] -= pp;
-          e[m] = 0.0;
-          deflated = 1;
-          break;
-        }
-        s = f / r;
-        c = g / r;
-        g = w[i + 1] - pp;
-        r = (w[i] - g) * s + 2.0 * c * b;
-        pp = s * r;
-        w[i + 1] = g + pp;
-        g = c * r - b;
-        rot_c[i] = c;
-        rot_s[i] = s;
-        ilast = i;
-      }
-      /* record the sweep's rotations; they are applied to QT after the QL
-       * iteration, all at once (the QL recurrence does not read QT) */
-      for (long ii = m - 1; ii >= ilast; ii--) {
-        if (nrot == caprot) {
-          caprot *= 2;
-          rot = (struct sep_rot *)realloc(rot, sizeof(struct sep_rot) * caprot);
-        }
-        rot[nrot].i = ii;
-        rot[nrot].c = rot_c[ii];
-        rot[nrot].s = rot_s[ii];
-        nrot++;
-      }
-      if (deflated) continue;
-      w[l] -= pp;
-      e[l] = g;
-      e[m] = 0.0;
-    }
-  }
-  /* apply the recorded rotations in order to the rows of QT; columns are
-   * independent, so column chunks go to separate threads */
-#pragma omp parallel for schedule(dynamic) if (n > 128)
-  for (long k0 = 0; k0 < n; k0 += 32) {
-    const long k1 = k0 + 32 < n ? k0 + 32 : n;
-    for (long t = 0; t < nrot; t++) {
-      const double cc = rot[t].c, ss = rot[t].s;
-      double *qa_row = Q + (rot[t].i + 1) * n, *qb_row = Q + rot[t].i * n;
-      for (long k = k0; k < k1; k++) {
-        const double qa = qa_row[k], qb = qb_row[k];
-        qa_row[k] = ss * qb + cc * qa;
-        qb_row[k] = cc * qb - ss * qa;
-      }
-    }
-  }
-  free(rot);
-  /* sort ascending */
-  long *idx = (long *)malloc(sizeof(long) * n);
-  for (long i = 0; i < n; i++) idx[i] = i;
-  for (long i = 1; i < n; i++) {
-    long x = idx[i], j = i;
-    while (j > 0 && w[idx[j - 1]] > w[x]) {
-      idx[j] = idx[j - 1];
-      j--;
-    }
-    idx[j] = x;
-  }
-  double *ws = (double *)malloc(sizeof(double) * n);
-  for (long k = 0; k < n; k++) {
-    ws[k] = w[idx[k]];
-    for (long r = 0; r < n; r++) A[r * n + k] = Q[idx[k] * n + r];
-  }
-  memcpy(w, ws, sizeof(double) * n);
-  free(ws);
-  free(idx);
-  free(Q);
-  free(v);
-  free(p);
-  free(e);
-  free(rot_c);
-  free(rot_s);
-  return status;
-}
-
 void sep_la_svd_jacobi(long n, const double *R, double *U, double *S, double *V) {
   /* columns of B (= R) orthogonalised by plane rotations in a round-robin
    * order whose pairs within a round are disjoint (parallel). */

@@ -8,19 +8,6 @@
 extern "C" {
 #endif
 
-/* In place upper Cholesky: A (n x n row-major, symmetric) = R^T R, R in the
- * upper triangle (lower zeroed).  Returns 0, or k+1 if the k-th pivot is not
- * positive. */
-int sep_la_chol_upper(long n, double *A);
-/* Rinv = R^{-1} for upper-triangular R (n x n row-major). */
-void sep_la_trinv_upper(long n, const double *R, double *Rinv);
-/* C = A B (n x n row-major). */
-void sep_la_matmul(long n, const double *A, const double *B, double *C);
-/* Symmetric eigen-decomposition by Householder tridiagonalisation and the
- * implicit QL algorithm: A (n x n row-major symmetric) is overwritten by the
- * eigenvectors (column k = vector k), w ascending.  Returns 0 or -1 if QL
- * did not converge. */
-int sep_la_symeig(long n, double *A, double *w);
 /* One-sided Jacobi SVD of a square R (n x n row-major): S descending,
  * U (n x n row-major) columns = left singular vectors (zero column for a
  * zero singular value), V (optional) columns = right singular vectors. */
