@@ -167,7 +167,9 @@ int sep_filter_batch(int ndots, sep_ctx *const *ctxs, const void *const *psi0_de
  * pass A (xy forward), pass B (z pencils with k^2), pass C (xy backward +
  * Newton epilogue) and one accumulation launch (K = 64 Newton vectors,
  * ms targets, 2 nvec real states); kernel_ms[4] = passes A+B+C timed as one
- * span.  No counterpart in the reference (it only prints clock() totals,
+ * span; kernel_ms[5] = the filter loop's fused kernel (pass C with the next
+ * step's pass A), 0 where the grid runs the passes separately.  kernel_ms
+ * holds 6 values.  No counterpart in the reference (it only prints clock() totals,
  * filter/main.c:63-66,91-100). */
 int sep_bench_hpsi(sep_ctx *ctx, int precision, long nvec, long ms, long iters,
                    double *kernel_ms);
@@ -177,7 +179,9 @@ int sep_bench_hpsi(sep_ctx *ctx, int precision, long nvec, long ms, long iters,
  * one Newton step per ring block, and around that block's accumulation
  * launch (one host wait per block).  sep_set_timing resets the sums;
  * sep_get_timing returns avg_ms[0..3] = passes A, B, C and the accumulation
- * of a full ring block, and samples[0..1] = the number of sampled steps and
+ * of a full ring block (where the loop runs pass C fused with the next
+ * step's pass A, avg_ms[0] = 0 and avg_ms[2] is the fused kernel), and
+ * samples[0..1] = the number of sampled steps and
  * of sampled full blocks.  No counterpart in the reference (it only prints
  * clock() totals, filter/main.c:63-66,91-100). */
 int sep_set_timing(sep_ctx *ctx, int on);

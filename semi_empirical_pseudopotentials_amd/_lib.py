@@ -366,7 +366,7 @@ class Context:
         self._chk(rc, "sep_filter_dev")
 
     def bench_hpsi(self, precision, nvec, ms, iters):
-        out = np.zeros(5)
+        out = np.zeros(6)
         rc = lib().sep_bench_hpsi(self._h, precision, nvec, ms, iters, _p(out))
         self._chk(rc, "sep_bench_hpsi")
         return out

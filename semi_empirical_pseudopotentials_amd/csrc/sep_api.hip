@@ -214,6 +214,7 @@ int sep_open(const sep_grid_desc *g, sep_ctx **out) {
   c->gd.kzz = tabs_d[2];
   c->gd.ekin = c->ekinmax * (ny_1 * nx_1 * nz_1);
   c->gd.force_lines[0] = c->gd.force_lines[1] = getenv("SEP_FORCE_LINE_KERNELS") != nullptr;
+  c->gd.no_fuse = getenv("SEP_SEPARATE_PASSES") != nullptr;
   *out = c;
   return SEP_OK;
 }
@@ -575,6 +576,7 @@ struct FilterRun {
   double *an_d = nullptr;
   R *ring = nullptr;
   bool sampled = false;
+  bool fused = false;  // split steps: tb carries the spectrum of the newest vector
 };
 
 __global__ void k_zero(double *x, long n) {
@@ -593,6 +595,7 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
                         long ns, const double *an, const double *zn, long nc, long ms, double vmin,
                         double dE) {
   const long N = c->gd.N;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
   f.c = c;
   f.N = N;
   f.ns = ns;
@@ -646,6 +649,13 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
   HIPCHK(c, hipStreamSynchronize(c->st));  // anT outlives the copy
   f.zn.assign(zn, zn + nc);
   f.j = 1;
+  // pass A of psi_0; every later pass A runs fused with the previous pass C
+  f.fused = nc > 1 && !c->gd.no_fuse && hpsi_fused_supported<R>(c->gd);
+  if (f.fused) {
+    VecView<R> v0{f.ring, f.ring + ss, nullptr, 2 * ss};
+    if (hpsi_fwd<R>(c->gd, v0, reinterpret_cast<cx<R> *>(c->tb), (int)f.nvec, c->st) != SEP_OK)
+      return fail(c, SEP_ERR_HIP, "filter: pass A launch failed");
+  }
   return SEP_OK;
 }
 
@@ -662,7 +672,13 @@ static int filter_step(FilterRun<R> &f) {
   // psi_j = dE_1 H psi_{j-1} - (2 + z_{j-1} + Vmin dE_1) psi_{j-1}
   const double beta = 2.0 + f.zn[j - 1] + f.vmin * f.dE_1;
   const bool sample = c->timing && sj == K / 2;
-  RCHK(apply_views<R>(c, in, out, (int)f.nvec, f.dE_1, beta, nullptr, sample ? c->tev : nullptr));
+  if (f.fused) {
+    if (hpsi_step<R>(c->gd, in, out, reinterpret_cast<cx<R> *>(c->tb), pot_of<R>(c), (int)f.nvec,
+                     f.dE_1, beta, j + 1 < f.nc, c->st, sample ? c->tev : nullptr) != SEP_OK)
+      return fail(c, SEP_ERR_HIP, "filter: H.psi step launch failed");
+  } else {
+    RCHK(apply_views<R>(c, in, out, (int)f.nvec, f.dE_1, beta, nullptr, sample ? c->tev : nullptr));
+  }
   f.sampled |= sample;
   if (sj == K - 1 || j == f.nc - 1) {
     if (f.sampled) hipEventRecord(c->tev[4], c->st);
@@ -672,7 +688,7 @@ static int filter_step(FilterRun<R> &f) {
       hipEventRecord(c->tev[5], c->st);
       HIPCHK(c, hipEventSynchronize(c->tev[5]));
       float t;
-      for (int q = 0; q < 3; q++) {
+      for (int q = f.fused ? 1 : 0; q < 3; q++) {  // fused: pass A runs inside pass C
         hipEventElapsedTime(&t, c->tev[q], c->tev[q + 1]);
         c->tsum[q] += t;
       }
@@ -1264,9 +1280,12 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
   VecView<R> in{ring, ring + N, nullptr, 2 * N};
   VecView<R> out{ring + S2 * N, ring + S2 * N + N, nullptr, 2 * N};
   cx<R> *tb = reinterpret_cast<cx<R> *>(c->tb);
-  hipEvent_t ev[6];
-  for (int i = 0; i < 6; i++) HIPCHK(c, hipEventCreate(&ev[i]));
-  double acc[5] = {0, 0, 0, 0, 0};
+  // the filter loop's split step (pass B, then pass C fused with the next
+  // pass A) where the grid supports it: ev[4..7] around pass B and the fused kernel
+  const bool fused = hpsi_fused_supported<R>(c->gd);
+  hipEvent_t ev[8];
+  for (int i = 0; i < 8; i++) HIPCHK(c, hipEventCreate(&ev[i]));
+  double acc[6] = {0, 0, 0, 0, 0, 0};
   int rc = SEP_OK;
   for (long it = -1; it < iters && rc == SEP_OK; it++) {  // it = -1: warm-up
     rc = hpsi_apply<R>(c->gd, in, out, tb, pot_of<R>(c), (int)nvec, 0.3, 1.7, nullptr, nullptr,
@@ -1275,7 +1294,9 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
       rc = accum_mfma_dev<R>(ring, S2 * N, N, (int)K, an_d, 0, (int)ms, c->states, (int)S2, N,
                              c->st);
     hipEventRecord(ev[4], c->st);
-    if (hipEventSynchronize(ev[4]) != hipSuccess) rc = SEP_ERR_HIP;
+    if (rc == SEP_OK && fused)
+      rc = hpsi_step<R>(c->gd, in, out, tb, pot_of<R>(c), (int)nvec, 0.3, 1.7, true, c->st, ev + 4);
+    if (hipEventSynchronize(ev[fused ? 7 : 4]) != hipSuccess) rc = SEP_ERR_HIP;
     if (it >= 0 && rc == SEP_OK) {
       float t;
       for (int k = 0; k < 4; k++) {
@@ -1284,12 +1305,16 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
       }
       hipEventElapsedTime(&t, ev[0], ev[3]);
       acc[4] += t;
+      if (fused) {
+        hipEventElapsedTime(&t, ev[6], ev[7]);
+        acc[5] += t;
+      }
     }
   }
-  for (int i = 0; i < 6; i++) hipEventDestroy(ev[i]);
+  for (int i = 0; i < 8; i++) hipEventDestroy(ev[i]);
   hipFree(an_d);
   if (rc != SEP_OK) return fail(c, rc, "bench launch failed");
-  for (int k = 0; k < 5; k++) kms[k] = acc[k] / (double)(iters > 0 ? iters : 1);
+  for (int k = 0; k < 6; k++) kms[k] = acc[k] / (double)(iters > 0 ? iters : 1);
   return SEP_OK;
 }
 
@@ -1312,7 +1337,7 @@ extern "C" int sep_autotune(sep_ctx *c, int precision, long nvec, double *ms_pla
     return SEP_ERR_ARG;
   if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
   HIPCHK(c, hipSetDevice(c->device));
-  double k[5];
+  double k[6];
   double tp = -1.0, tl = -1.0;
   const bool had = c->gd.force_lines[precision];
   if (plane_supported(c->gd.nx, c->gd.ny, precision)) {

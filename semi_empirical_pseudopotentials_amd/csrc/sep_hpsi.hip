@@ -685,6 +685,162 @@ __global__ void __launch_bounds__(T) k_plane_bwd_w(const cx<R> *tb, VecView<R> p
   if (epi.part) block_sum4<T>(s, epi.part, blockIdx.x);
 }
 
+// An int the compiler must treat as unknown: index and LDS address math
+// derived from it is recomputed where it is used instead of being kept live
+// in registers across the phases of a kernel.
+__device__ __forceinline__ int opaque_int(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// Pass C of Newton step j fused with pass A of step j+1 (whole-plane buffer,
+// NH == 1).  The epilogue's psi_j is stored (the ring slot the accumulation
+// reads) and, still in registers, transformed forward -- x rows in each row's
+// own buffer row, then y columns -- into plane z of the spectrum buffer,
+// which this block alone read at its start.  Saves pass A's read of psi_j;
+// the arithmetic is that of k_plane_bwd_w followed by k_plane_fwd_w.
+template <typename R, int NX, int T>
+__global__ void __launch_bounds__(T) k_plane_bwd_fwd_w(cx<R> *tb, VecView<R> pin, VecView<R> out,
+                                                       Epi<R> epi, long N, int nz,
+                                                       const cx<R> *twg) {
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, P = PL::P;
+  static_assert(64 % G == 0 && NX % 8 == 0, "lines must not straddle wavefronts");
+  static_assert(plane_chunks<R, NX>() == 1, "the fused kernel holds the whole plane in LDS");
+  using L = PlaneLds<R, NX, 1>;
+  constexpr int LDc = L::LDc;
+  constexpr int RX = T / G;
+  constexpr int NR = (NX + RX - 1) / RX;
+  extern __shared__ char smem_raw[];
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
+  cx<R> *tw = buf + NX * LDc;
+  const long vec = blockIdx.x / nz;
+  const long z = blockIdx.x % nz;
+  const long pbase = z * (long)NX * NX;
+  const int tl = threadIdx.x / G, g0 = threadIdx.x % G;
+  const int g = g0;
+  constexpr int r0b = Pass<NX, 1>::radix(0);
+  constexpr int rlb = Pass<NX, 1>::radix(P - 1);
+  constexpr int rlf = Pass<NX, -1>::radix(P - 1);
+  constexpr bool early = T < 1024;
+  using TwB = typename std::conditional<(T >= 1024), TwTable<NX, 1, R>, TwFor<NX, 1, R>>::type;
+  using TwF = typename std::conditional<(T >= 1024), TwTable<NX, -1, R>, TwFor<NX, -1, R>>::type;
+  TwB tcb;
+  TwF tcf;
+  // ---- inverse y FFT of the columns (pass C)
+  {
+    cx<R> w[NR][E];
+#pragma unroll
+    for (int cr = 0; cr < NR; cr++) {
+      const int cl = cr * RX + tl;
+      if (cl < NX) {
+        const cx<R> *src = tb + vec * N + pbase + (long)cl * NX;
+#pragma unroll
+        for (int b = 0; b < E / r0b; b++)
+#pragma unroll
+          for (int k = 0; k < r0b; k++) w[cr][b * r0b + k] = src[in_idx<NX, 1, 0>(g, b, k)];
+      }
+    }
+    load_tw<R, NX>(tw, twg);
+    __syncthreads();
+    tcb.init(g, tw);
+    tcf.init(g, tw);
+#pragma unroll
+    for (int cr = 0; cr < NR; cr++) {
+      const int cl = cr * RX + tl;
+      const bool act = cl < NX;
+      const int cs = act ? cl : 0;
+      auto at = [&](int i) -> cx<R> & { return buf[L::rowbase(i) + cs]; };
+      run_passes_c<NX, 1, 0, P, WaveSync>(w[cr], g, act, tcb, at);
+      WaveSync::sync();
+      write_out<NX, 1, P - 1>(w[cr], g, act, at);
+    }
+  }
+  __syncthreads();  // the transpose
+  // ---- rows: inverse x FFT, epilogue, store psi_j, forward x FFT of psi_j
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int g = opaque_int(g0);  // per-round index math (see opaque_int)
+    tcb.init(g, tw);
+    tcf.init(g, tw);
+    const int row = r * RX + tl;
+    const bool act = row < NX;
+    const int rs = act ? row : 0;
+    cx<R> p[E];
+    R V[E];
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / rlb; b++)
+#pragma unroll
+        for (int k = 0; k < rlb; k++) {
+          const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
+          if (early) p[b * rlb + k] = pin.ld(vec, gi);
+          V[b * rlb + k] = epi.pot[gi];
+        }
+    }
+    cx<R> x[E];
+    auto at0 = [&](int i) -> cx<R> & { return buf[L::rowbase(rs) + i]; };
+    read_in<NX, 1, 0>(x, g, act, at0);
+    auto at = [&](int i) -> cx<R> & { return buf[L::rowbase(rs) + L::xpos(i)]; };
+    run_passes_c<NX, 1, 0, P, WaveSync>(x, g, act, tcb, at);
+    if (act) {
+      if constexpr (!early) {
+#pragma unroll
+        for (int b = 0; b < E / rlb; b++)
+#pragma unroll
+          for (int k = 0; k < rlb; k++) {
+            const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
+            p[b * rlb + k] = pin.ld(vec, gi);
+          }
+      }
+#pragma unroll
+      for (int b = 0; b < E / rlb; b++)
+#pragma unroll
+        for (int k = 0; k < rlb; k++) {
+          const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
+          x[b * rlb + k] = epilogue(epi, x[b * rlb + k], p[b * rlb + k], V[b * rlb + k], s);
+          out.st(vec, gi, x[b * rlb + k]);
+        }
+    }
+    // psi_j's row back through the row buffer into the forward input order
+    WaveSync::sync();
+    write_out<NX, 1, P - 1>(x, g, act, at);
+    WaveSync::sync();
+    read_in<NX, -1, 0>(x, g, act, at);
+    run_passes_c<NX, -1, 0, P, WaveSync>(x, g, act, tcf, at);
+    WaveSync::sync();
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / rlf; b++)
+#pragma unroll
+        for (int k = 0; k < rlf; k++)
+          buf[L::rowbase(rs) + out_idx<NX, -1, P - 1>(g, b, k)] = x[b * rlf + k];
+    }
+  }
+  __syncthreads();  // the transpose
+  // ---- forward y FFT of the columns (pass A of step j+1) into plane z of tb
+#pragma unroll
+  for (int cr = 0; cr < NR; cr++) {
+    const int g = opaque_int(g0);
+    tcf.init(g, tw);
+    const int cl = cr * RX + tl;
+    const bool act = cl < NX;
+    const int cs = act ? cl : 0;
+    cx<R> y[E];
+    auto at = [&](int i) -> cx<R> & { return buf[L::rowbase(i) + cs]; };
+    read_in<NX, -1, 0>(y, g, act, at);
+    run_passes_c<NX, -1, 0, P, WaveSync>(y, g, act, tcf, at);
+    if (act) {
+      cx<R> *dst = tb + vec * N + pbase + (long)cs * NX;
+#pragma unroll
+      for (int b = 0; b < E / rlf; b++)
+#pragma unroll
+        for (int k = 0; k < rlf; k++) dst[out_idx<NX, -1, P - 1>(g, b, k)] = y[b * rlf + k];
+    }
+  }
+}
+
 // ================================================================= dispatch
 template <int n>
 constexpr bool plane_colmajor() {
@@ -730,6 +886,9 @@ struct HOps {
                         const cx<R> *tw, hipStream_t st);
   static bool plane_bwd(int nxy, const cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
                         long N, int nz, int nvec, const cx<R> *tw, hipStream_t st, long *nblk);
+  // pass C fused with the next pass A; false where the plane needs chunks
+  static bool plane_bwd_fwd(int nxy, cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
+                            long N, int nz, int nvec, const cx<R> *tw, hipStream_t st);
 };
 
 #define SEP_LINE_SIZES(X) X(16) X(24) X(32) X(36) X(48) X(64) X(80) X(96) X(128) X(160)
@@ -881,6 +1040,54 @@ static void plane_bwd_launch(const cx<R> *tb, VecView<R> pin, VecView<R> out, Ep
   }
 }
 
+template <typename R, int n>
+static bool plane_bwd_fwd_launch(cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi, long N,
+                                 int nz, int nvec, const cx<R> *tw, hipStream_t st) {
+  if constexpr (plane_colmajor<n>() && plane_chunks<R, n>() == 1) {
+    constexpr int T = plane_threads<R, n>();
+    constexpr size_t lds = plane_lds<R, n>();
+    static_assert(lds <= 160 * 1024, "plane buffer exceeds the LDS of a gfx950 workgroup");
+    static bool attr = false;
+    if (!attr) {
+      hipFuncSetAttribute((const void *)k_plane_bwd_fwd_w<R, n, T>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      attr = true;
+    }
+    if (nvec == 0) return true;  // attribute only (before graph capture)
+    hipLaunchKernelGGL((k_plane_bwd_fwd_w<R, n, T>), dim3((unsigned)(nz * nvec)), dim3(T), lds,
+                       st, tb, pin, out, epi, N, nz, tw);
+    return true;
+  }
+  return false;
+}
+
+template <>
+bool HOps<float>::plane_bwd_fwd(int nxy, cx<float> *tb, VecView<float> pin, VecView<float> out,
+                                Epi<float> epi, long N, int nz, int nvec, const cx<float> *tw,
+                                hipStream_t st) {
+  switch (nxy) {
+#define C(n) \
+  case n:    \
+    return plane_bwd_fwd_launch<float, n>(tb, pin, out, epi, N, nz, nvec, tw, st);
+    SEP_PLANE_SIZES_F32(C)
+#undef C
+  }
+  return false;
+}
+template <>
+bool HOps<double>::plane_bwd_fwd(int nxy, cx<double> *tb, VecView<double> pin,
+                                 VecView<double> out, Epi<double> epi, long N, int nz, int nvec,
+                                 const cx<double> *tw, hipStream_t st) {
+  switch (nxy) {
+#define C(n) \
+  case n:    \
+    return plane_bwd_fwd_launch<double, n>(tb, pin, out, epi, N, nz, nvec, tw, st);
+    SEP_PLANE_SIZES_F64(C)
+#undef C
+  }
+  return false;
+}
+
 template <>
 bool HOps<float>::plane_fwd(int nxy, VecView<float> in, cx<float> *tb, long N, int nz, int nvec,
                             const cx<float> *tw, hipStream_t st) {
@@ -1026,6 +1233,67 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 
+// Split Newton step: the filter loop runs pass A once on psi_0 (hpsi_fwd),
+// then per step pass B and pass C fused with the next step's pass A
+// (hpsi_step with fwd_next), the spectrum of psi_j staying in tb between
+// steps.  Available where the plane kernels hold a whole plane in LDS and
+// both planes of the vectors exist (the fused kernel keeps the imaginary
+// plane that a real-only view would drop).
+template <typename R>
+bool hpsi_fused_supported(const GridDev &gd) {
+  const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
+  if (gd.force_lines[prec] || !plane_supported(gd.nx, gd.ny, prec) || gd.pden > 0.0) return false;
+  VecView<R> none{nullptr, nullptr, nullptr, 0};
+  Epi<R> epi{nullptr, 0.0, 0.0, nullptr};
+  return HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, nullptr, nullptr);
+}
+
+template <typename R>
+int hpsi_fwd(const GridDev &gd, const VecView<R> &in, cx<R> *tb, int nvec, hipStream_t st) {
+  const cx<R> *twx = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
+  if (!hpsi_fused_supported<R>(gd) || !in.im) return SEP_ERR_UNSUPPORTED;
+  if (!HOps<R>::plane_fwd(gd.nx, in, tb, gd.N, gd.nz, nvec, twx, st)) return SEP_ERR_UNSUPPORTED;
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
+// tb holds the spectrum of in (psi_{j-1}); out <- alpha (T + V) in - beta in
+// and, with fwd_next, tb <- the spectrum of out.  Events as hpsi_apply's
+// ev[1..3] (before pass B, after pass B, after the fused kernel); ev[0] is
+// not recorded.
+template <typename R>
+int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
+              const R *pot, int nvec, double alpha, double beta, bool fwd_next, hipStream_t st,
+              hipEvent_t *ev) {
+  const int nx = gd.nx, ny = gd.ny, nz = gd.nz;
+  const long N = gd.N;
+  const cx<R> *twx = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
+  const cx<R> *twz = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[2] : gd.tw32[2]);
+  if (!hpsi_fused_supported<R>(gd) || !in.im || !out.im) return SEP_ERR_UNSUPPORTED;
+  if (ev) hipEventRecord(ev[1], st);
+  const bool cm = plane_is_colmajor(nx);
+  if (!HOps<R>::pencil(nz, tb, N, cm ? ny : nx, cm ? nx : ny, nvec, twz, cm ? gd.kyy : gd.kxx,
+                       cm ? gd.kxx : gd.kyy, gd.kzz, gd.ekin, 1.0, 0.0, st))
+    return SEP_ERR_UNSUPPORTED;
+  if (ev) hipEventRecord(ev[2], st);
+  Epi<R> epi{pot, alpha, beta, nullptr};
+  const bool ok = fwd_next ? HOps<R>::plane_bwd_fwd(nx, tb, in, out, epi, N, nz, nvec, twx, st)
+                           : HOps<R>::plane_bwd(nx, tb, in, out, epi, N, nz, nvec, twx, st, nullptr);
+  if (!ok) return SEP_ERR_UNSUPPORTED;
+  if (ev) hipEventRecord(ev[3], st);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+template bool hpsi_fused_supported<double>(const GridDev &);
+template bool hpsi_fused_supported<float>(const GridDev &);
+template int hpsi_fwd<double>(const GridDev &, const VecView<double> &, cx<double> *, int,
+                              hipStream_t);
+template int hpsi_fwd<float>(const GridDev &, const VecView<float> &, cx<float> *, int, hipStream_t);
+template int hpsi_step<double>(const GridDev &, const VecView<double> &, const VecView<double> &,
+                               cx<double> *, const double *, int, double, double, bool,
+                               hipStream_t, hipEvent_t *);
+template int hpsi_step<float>(const GridDev &, const VecView<float> &, const VecView<float> &,
+                              cx<float> *, const float *, int, double, double, bool, hipStream_t,
+                              hipEvent_t *);
+
 // Kernel attributes of this grid's H.psi kernels, set outside any stream
 // capture (hipFuncSetAttribute is not a stream operation).
 template <typename R>
@@ -1037,6 +1305,7 @@ int hpsi_prepare(const GridDev &gd) {
   Epi<R> epi{nullptr, 0.0, 0.0, nullptr};
   HOps<R>::plane_fwd(gd.nx, none, nullptr, gd.N, gd.nz, 0, tw, nullptr);
   HOps<R>::plane_bwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr, nullptr);
+  HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 template int hpsi_prepare<double>(const GridDev &);

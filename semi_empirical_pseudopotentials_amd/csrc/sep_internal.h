@@ -23,6 +23,9 @@ struct GridDev {
   // kernels even where a plane kernel exists (sep_autotune, or the test hook
   // SEP_FORCE_LINE_KERNELS)
   bool force_lines[2];
+  // the filter loop runs pass C and the next pass A as separate kernels
+  // (test hook SEP_SEPARATE_PASSES) instead of the fused plane kernel
+  bool no_fuse;
 };
 
 bool hop_supported(int n);
@@ -34,6 +37,17 @@ template <typename R>
 int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
                const R *pot, int nvec, double alpha, double beta, double *part, long *nblk,
                hipStream_t st, hipEvent_t *ev = nullptr);
+// Split Newton step (sep_hpsi.hip): pass A alone, then per step pass B and
+// pass C fused with the next step's pass A (fwd_next); tb carries the
+// spectrum of the newest vector between calls.
+template <typename R>
+bool hpsi_fused_supported(const GridDev &gd);
+template <typename R>
+int hpsi_fwd(const GridDev &gd, const VecView<R> &in, cx<R> *tb, int nvec, hipStream_t st);
+template <typename R>
+int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
+              const R *pot, int nvec, double alpha, double beta, bool fwd_next, hipStream_t st,
+              hipEvent_t *ev = nullptr);
 
 // ---- element-wise / reduction kernels (sep_kernels.hip)
 int build_potential_dev(const GridDev &gd, double xmin, double ymin, double zmin, double dx,
