@@ -574,6 +574,7 @@ struct FilterRun {
   double vmin = 0.0, dE_1 = 0.0;
   std::vector<double> zn;
   double *an_d = nullptr;
+  float *cf_d = nullptr;  // complex64 filter: coefficients packed for accum_mfma_packed_dev
   R *ring = nullptr;
   bool sampled = false;
   bool fused = false;  // split steps: tb carries the spectrum of the newest vector
@@ -646,7 +647,15 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
     for (long j = 0; j < nc; j++) anT[j * ms + ie] = an[ie * nc + j];
   HIPCHK(c, hipMalloc(&f.an_d, sizeof(double) * ms * nc));
   HIPCHK(c, hipMemcpyAsync(f.an_d, anT.data(), sizeof(double) * ms * nc, hipMemcpyHostToDevice, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));  // anT outlives the copy
+  std::vector<float> cf;
+  if (sizeof(R) == 4) {
+    cf.resize(accum_pack_len(nc, (int)ms));
+    accum_pack_f32(an, nc, (int)ms, cf.data());
+    HIPCHK(c, hipMalloc(&f.cf_d, sizeof(float) * cf.size()));
+    HIPCHK(c, hipMemcpyAsync(f.cf_d, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice,
+                             c->st));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));  // anT and cf outlive the copies
   f.zn.assign(zn, zn + nc);
   f.j = 1;
   // pass A of psi_0; every later pass A runs fused with the previous pass C
@@ -657,6 +666,26 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
       return fail(c, SEP_ERR_HIP, "filter: pass A launch failed");
   }
   return SEP_OK;
+}
+
+// acc += the block of cnt stored Newton vectors starting at Newton index j0
+// (filter/filter.c:74-78): the packed-coefficient kernel for the complex64
+// filter where the shape allows it, else the general one.
+template <typename R>
+static int accum_block(FilterRun<R> &f, long sstride, int cnt, long j0) {
+  sep_ctx *c = f.c;
+  if constexpr (sizeof(R) == 4) {
+    if (f.cf_d) {
+      const int rc = accum_mfma_packed_dev(reinterpret_cast<const float *>(f.ring), f.slot,
+                                           sstride, cnt, f.cf_d, f.nc, j0, (int)f.ms, c->states,
+                                           (int)f.ns, f.N, c->st);
+      if (rc != SEP_ERR_UNSUPPORTED)
+        return rc == SEP_OK ? SEP_OK : fail(c, rc, "accumulation launch failed");
+    }
+  }
+  const int rc = accum_mfma_dev<R>(f.ring, f.slot, sstride, cnt, f.an_d, j0, (int)f.ms, c->states,
+                                   (int)f.ns, f.N, c->st);
+  return rc == SEP_OK ? SEP_OK : fail(c, rc, "accumulation launch failed");
 }
 
 // Newton step f.j (filter/filter.c:70-78, hnorm.c:19-24).  No host
@@ -682,8 +711,7 @@ static int filter_step(FilterRun<R> &f) {
   f.sampled |= sample;
   if (sj == K - 1 || j == f.nc - 1) {
     if (f.sampled) hipEventRecord(c->tev[4], c->st);
-    RCHK(accum_mfma_dev<R>(f.ring, slot, ss, (int)(sj + 1), f.an_d, j - sj, (int)f.ms,
-                           c->states, (int)f.ns, N, c->st));
+    RCHK(accum_block<R>(f, ss, (int)(sj + 1), j - sj));
     if (f.sampled) {
       hipEventRecord(c->tev[5], c->st);
       HIPCHK(c, hipEventSynchronize(c->tev[5]));
@@ -709,12 +737,12 @@ template <typename R>
 static int filter_finish(FilterRun<R> &f, int rc) {
   sep_ctx *c = f.c;
   if (rc == SEP_OK && f.nc == 1)
-    rc = accum_mfma_dev<R>(f.ring, f.slot, f.K * f.N, 1, f.an_d, 0, (int)f.ms, c->states,
-                           (int)f.ns, f.N,
-                           c->st);
+    rc = accum_block<R>(f, f.K * f.N, 1, 0);
   hipStreamSynchronize(c->st);
   if (f.an_d) hipFree(f.an_d);
+  if (f.cf_d) hipFree(f.cf_d);
   f.an_d = nullptr;
+  f.cf_d = nullptr;
   if (rc != SEP_OK) return rc;
   return normalize_dev(c, c->states, f.ns * f.ms);  // filter.c:33
 }
@@ -1277,6 +1305,13 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
   double *an_d = nullptr;
   HIPCHK(c, hipMalloc(&an_d, sizeof(double) * ms * K));
   HIPCHK(c, hipMemcpy(an_d, an.data(), sizeof(double) * ms * K, hipMemcpyHostToDevice));
+  float *cf_d = nullptr;  // the complex64 filter's packed coefficients (values immaterial)
+  if (sizeof(R) == 4) {
+    std::vector<float> cf(accum_pack_len(K, (int)ms));
+    accum_pack_f32(an.data(), K, (int)ms, cf.data());
+    HIPCHK(c, hipMalloc(&cf_d, sizeof(float) * cf.size()));
+    HIPCHK(c, hipMemcpy(cf_d, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice));
+  }
   VecView<R> in{ring, ring + N, nullptr, 2 * N};
   VecView<R> out{ring + S2 * N, ring + S2 * N + N, nullptr, 2 * N};
   cx<R> *tb = reinterpret_cast<cx<R> *>(c->tb);
@@ -1290,9 +1325,15 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
   for (long it = -1; it < iters && rc == SEP_OK; it++) {  // it = -1: warm-up
     rc = hpsi_apply<R>(c->gd, in, out, tb, pot_of<R>(c), (int)nvec, 0.3, 1.7, nullptr, nullptr,
                        c->st, ev);
-    if (rc == SEP_OK)
-      rc = accum_mfma_dev<R>(ring, S2 * N, N, (int)K, an_d, 0, (int)ms, c->states, (int)S2, N,
-                             c->st);
+    if (rc == SEP_OK) {
+      rc = SEP_ERR_UNSUPPORTED;
+      if (cf_d)
+        rc = accum_mfma_packed_dev(reinterpret_cast<const float *>(ring), S2 * N, N, (int)K, cf_d,
+                                   K, 0, (int)ms, c->states, (int)S2, N, c->st);
+      if (rc == SEP_ERR_UNSUPPORTED)
+        rc = accum_mfma_dev<R>(ring, S2 * N, N, (int)K, an_d, 0, (int)ms, c->states, (int)S2, N,
+                               c->st);
+    }
     hipEventRecord(ev[4], c->st);
     if (rc == SEP_OK && fused)
       rc = hpsi_step<R>(c->gd, in, out, tb, pot_of<R>(c), (int)nvec, 0.3, 1.7, true, c->st, ev + 4);
@@ -1313,6 +1354,7 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
   }
   for (int i = 0; i < 8; i++) hipEventDestroy(ev[i]);
   hipFree(an_d);
+  if (cf_d) hipFree(cf_d);
   if (rc != SEP_OK) return fail(c, rc, "bench launch failed");
   for (int k = 0; k < 6; k++) kms[k] = acc[k] / (double)(iters > 0 ? iters : 1);
   return SEP_OK;

@@ -91,6 +91,15 @@ template <typename R>
 int accum_mfma_dev(const R *ring, long slot, long sstride, int cnt, const double *anT_d, long j0,
                    int ms, double *acc, int ns, long N, hipStream_t st);
 
+// The same for the complex64 filter with the coefficients packed as FP32
+// (accum_pack_f32 from an[ms][nc]; accum_pack_len floats); returns
+// SEP_ERR_UNSUPPORTED (nothing launched) where the shape does not allow it
+// (N % 256, slot / sstride % 4, j0 % 16).
+long accum_pack_len(long nc, int ms);
+void accum_pack_f32(const double *an, long nc, int ms, float *cf);
+int accum_mfma_packed_dev(const float *ring, long slot, long sstride, int cnt, const float *cf_d,
+                          long nc, long j0, int ms, double *acc, int ns, long N, hipStream_t st);
+
 // ---- GEMMs (sep_gemm.hip); all operands state-major (row i = one state)
 // C[i][j] = sum_g A[i][g] B[j][g]   (m x mb), deterministic split-K
 template <typename R>
