@@ -8,10 +8,14 @@
  * Options (the reference ignores argv; these are extensions):
  *   --dir D            deck / output directory (default .)
  *   --seed S           ran_nrc seed (overrides seed.par; 0 = clock as main.c:79)
- *   --precision P      fp64 (default) or fp32: complex64 filter, FP32-MFMA ortho,
- *                      FP64 Rayleigh-Ritz and one FP64 residual-correction pass
+ *   --precision P      fp64 (default) or fp32: complex64 filter, FP64-Gram ortho,
+ *                      FP64 Rayleigh-Ritz and one FP64 preconditioned correction pass
  *   --device N         HIP device ordinal
  *   --psi-filt         also write psi-filt-0-<jns>.dat (filter/filter.c:44-48)
+ *   --resume-filt      skip the random states and the filter: read the filtered
+ *                      states from psi-filt-0-<jns>.dat of an earlier --psi-filt
+ *                      run and continue at ortho (the reading that
+ *                      filter/main.c:102-106 holds commented out)
  * Failures write the message to a file named `error` and exit 1, as the
  * reference's nerror() does (filter/nerror.c:9-16).
  */
@@ -79,7 +83,7 @@ static void write_func(const double *an0, const double *zn, long nc, double dE, 
 
 int main(int argc, char **argv) {
   long seed_opt = 0;
-  int have_seed_opt = 0, prec = SEP_PREC_FP64, device = 0, psi_filt = 0;
+  int have_seed_opt = 0, prec = SEP_PREC_FP64, device = 0, psi_filt = 0, resume = 0;
   for (int i = 1; i < argc; i++) {
     if (!strcmp(argv[i], "--dir") && i + 1 < argc)
       g_dir = argv[++i];
@@ -98,6 +102,8 @@ int main(int argc, char **argv) {
       device = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--psi-filt"))
       psi_filt = 1;
+    else if (!strcmp(argv[i], "--resume-filt"))
+      resume = 1;
   }
   stamp();
   separation();
@@ -192,50 +198,70 @@ int main(int argc, char **argv) {
   FILE *pc = out_file("coeff.dat", "w");
   if (pc) fclose(pc);
   write_func(an, zn, nc, dE, Vmin, dt, el[0]);
-  /* random states (main.c:79-89) */
-  long idum;
-  long s = have_seed_opt ? seed_opt : (d.has_seed ? d.seed : 0);
-  if (s == 0) {
-    struct timeval tv;
-    gettimeofday(&tv, NULL);
-    srandom((unsigned)(tv.tv_sec ^ tv.tv_usec));
-    idum = -random();
-  } else {
-    idum = s < 0 ? s : -s;
-  }
-  printf("seed = %ld\n", idum);
-  fflush(stdout);
-  double *psi0 = (double *)malloc(sizeof(double) * ns * N);
-  sep_rng rng;
-  sep_rng_reset(&rng);
-  for (long j = 0; j < ns; j++) sep_init_psi(&rng, &idum, N, d.dv, psi0 + j * N);
-  /* filter (main.c:91-100) */
   const long mstot = ms * ns;
-  double *ene = (double *)malloc(sizeof(double) * mstot);
-  double *psif = psi_filt ? (double *)malloc(sizeof(double) * mstot * N) : NULL;
-  t0 = wall();
-  c0 = (double)clock();
-  CHECK(sep_filter(ctx, psi0, ns, an, zn, nc, ms, Vmin, dE, prec, psif, ene));
-  printf("done calculating filter, CPU time (sec) %g, wall run time (sec) %g\n",
-         ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
-  for (long jns = 0; jns < ns; jns++) { /* filter.c:38-48 */
-    char name[64];
-    snprintf(name, sizeof name, "eval-filt-0-%ld.dat", jns);
-    FILE *pf = out_file(name, "w");
-    if (pf) {
-      for (long j = 0; j < ms; j++) fprintf(pf, "%ld %.16g %.16g\n", j, ene[jns * ms + j], el[j]);
-      fclose(pf);
-    }
-    if (psif) {
+  double *psi0 = NULL, *ene = NULL;
+  if (resume) { /* the filtered states of an earlier run, ns files of ms states */
+    double *psitot = (double *)malloc(sizeof(double) * mstot * N);
+    if (!psitot) die("psi-filt buffer");
+    for (long jns = 0; jns < ns; jns++) {
+      char name[64];
       snprintf(name, sizeof name, "psi-filt-0-%ld.dat", jns);
-      pf = out_file(name, "wb");
+      FILE *pf = out_file(name, "rb");
+      if (!pf) die("--resume-filt: a psi-filt-0-<jns>.dat file is missing");
+      size_t got = fread(psitot + jns * ms * N, sizeof(double), (size_t)(ms * N), pf);
+      int extra = fgetc(pf) != EOF;
+      fclose(pf);
+      if (got != (size_t)(ms * N) || extra) die("--resume-filt: psi-filt file of the wrong size");
+    }
+    CHECK(sep_states_upload(ctx, psitot, mstot));
+    free(psitot);
+    printf("read %ld filtered states from psi-filt-0-*.dat\n", mstot);
+    fflush(stdout);
+  } else {
+    /* random states (main.c:79-89) */
+    long idum;
+    long s = have_seed_opt ? seed_opt : (d.has_seed ? d.seed : 0);
+    if (s == 0) {
+      struct timeval tv;
+      gettimeofday(&tv, NULL);
+      srandom((unsigned)(tv.tv_sec ^ tv.tv_usec));
+      idum = -random();
+    } else {
+      idum = s < 0 ? s : -s;
+    }
+    printf("seed = %ld\n", idum);
+    fflush(stdout);
+    psi0 = (double *)malloc(sizeof(double) * ns * N);
+    sep_rng rng;
+    sep_rng_reset(&rng);
+    for (long j = 0; j < ns; j++) sep_init_psi(&rng, &idum, N, d.dv, psi0 + j * N);
+    /* filter (main.c:91-100) */
+    ene = (double *)malloc(sizeof(double) * mstot);
+    double *psif = psi_filt ? (double *)malloc(sizeof(double) * mstot * N) : NULL;
+    t0 = wall();
+    c0 = (double)clock();
+    CHECK(sep_filter(ctx, psi0, ns, an, zn, nc, ms, Vmin, dE, prec, psif, ene));
+    printf("done calculating filter, CPU time (sec) %g, wall run time (sec) %g\n",
+           ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
+    for (long jns = 0; jns < ns; jns++) { /* filter.c:38-48 */
+      char name[64];
+      snprintf(name, sizeof name, "eval-filt-0-%ld.dat", jns);
+      FILE *pf = out_file(name, "w");
       if (pf) {
-        fwrite(psif + jns * ms * N, sizeof(double), ms * N, pf);
+        for (long j = 0; j < ms; j++) fprintf(pf, "%ld %.16g %.16g\n", j, ene[jns * ms + j], el[j]);
         fclose(pf);
       }
+      if (psif) {
+        snprintf(name, sizeof name, "psi-filt-0-%ld.dat", jns);
+        pf = out_file(name, "wb");
+        if (pf) {
+          fwrite(psif + jns * ms * N, sizeof(double), ms * N, pf);
+          fclose(pf);
+        }
+      }
     }
+    free(psif);
   }
-  free(psif);
   /* ortho (main.c:109-114) */
   t0 = wall();
   c0 = (double)clock();

@@ -52,3 +52,28 @@ def test_driver_in13_reference(sep, tmp_path):
     pairs = match_levels(ref_e, ref_s, got_e, 1e-6)
     assert len(pairs) >= 40
     assert np.abs(pairs[:, 0] - pairs[:, 1]).max() < 1e-8
+
+
+def test_driver_resume_from_filtered_states(sep, small_deck, tmp_path):
+    """--psi-filt writes the filtered states; --resume-filt in a copy of that
+    directory skips the filter and reproduces eval.dat and psi.dat exactly."""
+    print("case driver resume", flush=True)
+    first = str(tmp_path / "first")
+    _run_filter(small_deck, first, "--seed", "-1001", "--psi-filt")
+    again = str(tmp_path / "again")
+    shutil.copytree(first, again)
+    for f in ("eval.dat", "psi.dat"):
+        os.remove(os.path.join(again, f))
+    from semi_empirical_pseudopotentials_amd._lib import FILTER_BIN
+    r = subprocess.run([FILTER_BIN, "--dir", again, "--resume-filt"], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "filtered states from psi-filt" in r.stdout
+    assert "done calculating filter" not in r.stdout
+    for f in ("eval.dat", "psi.dat"):
+        assert open(os.path.join(first, f), "rb").read() == open(os.path.join(again, f), "rb").read()
+    os.remove(os.path.join(again, "psi-filt-0-1.dat"))
+    r = subprocess.run([FILTER_BIN, "--dir", again, "--resume-filt"], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 1
+    assert "psi-filt" in open(os.path.join(again, "error")).read()
