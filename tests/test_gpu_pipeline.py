@@ -78,10 +78,15 @@ def test_filter_fp64(sep, oracle, setup, ns):
     assert np.abs(ene - ene_ref).max() < 1e-10
 
 
-def test_filter_fp32(sep, oracle, setup):
-    """complex64 recurrence with FP64 accumulators: close to the FP64 oracle."""
+@pytest.mark.parametrize("nc,ms", [(96, 4), (37, 5), (13, 33)])
+def test_filter_fp32(sep, oracle, setup, nc, ms):
+    """complex64 recurrence with FP64 accumulators: close to the FP64 oracle.
+    Newton lengths that are not a multiple of 16 (the packed accumulation's
+    partial last row group) and more than 32 targets (a second target group
+    of one) included."""
     d, grid, V, ksq = setup
-    N, nc, ms, ns = V.size, 96, 4, 3
+    print("case filter_fp32", nc, ms, flush=True)
+    N, ns = V.size, 3
     dE, zn, el, an = _coeffs(oracle, d, V, ksq, grid, nc, ms)
     psi0 = _random_states(oracle, ns, N, d["dv"])
     ref = np.concatenate([oracle.normalize_all(
