@@ -189,8 +189,10 @@ int sep_set_timing(sep_ctx *ctx, int on);
  * resident random vectors with the plane kernels (when the line length has
  * them) and with the row + strided-line kernels, and keeps the faster family
  * for this context and precision.  ms_plane (-1 when unavailable) and
- * ms_lines (optional) get the measured A+B+C times.  No counterpart in the
- * reference (FFTW plans with FFTW_ESTIMATE, filter/filter.c:22-23).  Uses the
+ * ms_lines (optional) get the measured time of one Newton step's H.psi
+ * (plane: pass B + the fused pass C / next pass A where the grid has it).
+ * No counterpart in the reference (FFTW plans with FFTW_ESTIMATE,
+ * filter/filter.c:22-23).  Uses the
  * context's state and ring buffers as scratch: call it before filtering. */
 int sep_autotune(sep_ctx *ctx, int precision, long nvec, double *ms_plane, double *ms_lines);
 int sep_get_timing(sep_ctx *ctx, double *avg_ms, long *samples);

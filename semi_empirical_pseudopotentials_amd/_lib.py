@@ -475,7 +475,7 @@ class Context:
         return int(kept[0]), S
 
     def refine(self, ev):
-        """One FP64 residual-correction pass on the resident Ritz pairs."""
+        """One FP64 kinetically preconditioned correction pass on the resident Ritz pairs."""
         ev = np.array(ev, dtype=np.float64)
         sg = np.empty_like(ev)
         self._chk(lib().sep_refine(self._h, ev.size, _p(ev), _p(sg)), "sep_refine")

@@ -1370,9 +1370,10 @@ extern "C" int sep_bench_hpsi(sep_ctx *c, int precision, long nvec, long ms, lon
   return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
 }
 
-// Kernel-family choice per grid and precision: time passes A+B+C of the
-// plane kernels and of the row + strided-line kernels on nvec resident
-// random vectors (bench_impl) and keep the faster.
+// Kernel-family choice per grid and precision: time one Newton step's H.psi
+// with the plane kernels (pass B + the fused pass C / next pass A where the
+// grid has it, else A + B + C) and with the row + strided-line kernels
+// (A + B + C) on nvec resident random vectors (bench_impl); keep the faster.
 extern "C" int sep_autotune(sep_ctx *c, int precision, long nvec, double *ms_plane,
                             double *ms_lines) {
   if (!c || nvec < 1 || (precision != SEP_PREC_FP64 && precision != SEP_PREC_FP32))
@@ -1390,7 +1391,9 @@ extern "C" int sep_autotune(sep_ctx *c, int precision, long nvec, double *ms_pla
       c->gd.force_lines[precision] = had;
       return rc;
     }
-    tp = k[4];
+    // the filter loop's plane step: pass B + the fused pass C / next pass A
+    // where the grid has it, else passes A + B + C
+    tp = k[5] > 0.0 && !c->gd.no_fuse ? k[1] + k[5] : k[4];
   }
   c->gd.force_lines[precision] = true;
   const int rc = precision == SEP_PREC_FP64 ? bench_impl<double>(c, nvec, 1, 5, k)
