@@ -128,7 +128,7 @@ def precision_check(sep, decks, read_deck, np):
 
 
 def batched_uneven(sep, decks, read_deck, np, torch, steps=1):
-    """SURVEY config 4: 8 CdSe dots, diameters seeded-uniform in 2-5 nm, each
+    """Batched uneven grids: 8 CdSe dots, diameters seeded-uniform in 2-5 nm, each
     on the smallest of 64/96/128/160^3 with spacing <= 0.85 bohr, Newton
     length 2048, 16 targets x 8 random states each, complex64 filter.  Times
     one filter pass of all dots: one dot after another, interleaved on their
@@ -192,7 +192,7 @@ def batched_uneven(sep, decks, read_deck, np, torch, steps=1):
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         out[name] = {"s": round(t / steps, 3), "hpsi_per_s": round(hpsi / t, 1)}
-        log(f"config 4 {name}: {t / steps:.3f} s, {hpsi / t:.0f} H.psi/s")
+        log(f"batched uneven grids {name}: {t / steps:.3f} s, {hpsi / t:.0f} H.psi/s")
     for o in dots:
         o["ctx"].close()
     return out
@@ -202,7 +202,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the filter "
-              "would shard random states across GPUs, see DESIGN.md", file=sys.stderr)
+              "would shard its random states across GPUs, one filter per GPU with no "
+              "communication", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
