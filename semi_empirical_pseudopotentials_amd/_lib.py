@@ -12,6 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libsep.so")
 FILTER_BIN = os.path.join(_HERE, "bin", "filter")
+COUPLINGS_BIN = os.path.join(_HERE, "bin", "couplings")
 
 SEP_OK = 0
 SEP_PREC_FP64 = 0

@@ -1,6 +1,8 @@
-"""Merging filter runs into one eval.par / psi.par set (combine.py, after
-the reference's combine_filter_outputs.py): sort order, renumbering, block
-mapping, text format and every refusal, on small synthetic runs."""
+"""Tooling around the filter's outputs, on the CPU: merging filter runs into
+one eval.par / psi.par set (combine.py, after the reference's
+combine_filter_outputs.py) -- sort order, renumbering, block mapping, text
+format and every refusal, on small synthetic runs -- and the couplings
+driver's refusal of malformed options."""
 import os
 
 import numpy as np
@@ -93,3 +95,21 @@ def test_cli(tmp_path, capsys):
     assert "error:" in capsys.readouterr().err
     with pytest.raises(SystemExit):
         C.main([str(tmp_path / "r")])  # neither --grid-source nor --ngrid
+
+
+def test_couplings_driver_rejects_bad_options(tmp_path):
+    """bin/couplings refuses malformed options before touching a device,
+    writing the reason to `error` as the filter driver does."""
+    import subprocess
+    from semi_empirical_pseudopotentials_amd._lib import COUPLINGS_BIN
+    for args, msg in ((["--fermi-energy", "np.float64(0.1)"], "needs a number"),
+                      (["--state-type", "both"], "adiabatic or diabatic"),
+                      (["--frobnicate"], "unknown option")):
+        r = subprocess.run([COUPLINGS_BIN, "--dir", str(tmp_path), *args], capture_output=True,
+                           text=True, timeout=60)
+        assert r.returncode == 1
+        assert msg in open(tmp_path / "error").read()
+    # a directory without a deck
+    r = subprocess.run([COUPLINGS_BIN, "--dir", str(tmp_path)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1
