@@ -353,9 +353,11 @@ def main():
     # loop fuses pass C with the next step's pass A (the live timing then
     # reports pass A as 0), that kernel reads T, psi_{j-1}, V and writes
     # psi_j and the next T: (4c + r) N nvec, and a step moves N (6c + r).
-    def roof(live, c, r):
+    def roof(live, c, r, lines=False):
+        # lines: the row + strided-line family, whose fused step's spectral
+        # part is y lines, z pencils and y lines (3 x 2c per point)
         if live[0] <= 0.0:
-            per = {"B": 2 * c * N * nvec, "CA": (4 * c + r) * N * nvec}
+            per = {"B": (6 if lines else 2) * c * N * nvec, "CA": (4 * c + r) * N * nvec}
             tms = {"B": live[1], "CA": live[2]}
         else:
             per = {"A": 2 * c * N * nvec, "B": 2 * c * N * nvec, "C": (3 * c + r) * N * nvec}
@@ -454,7 +456,10 @@ def main():
         "complex_hpsi_per_s": round(v32 / 2, 1),
         "roofline": roofline,
         "roofline_passes_fp32": r32,
-        "roofline_passes_fp64": roof(live64, 16, 8) if live64 is not None else None,
+        # complex128 plane kernels exist up to 96 points per line; 128 runs the
+        # row + line family
+        "roofline_passes_fp64": (roof(live64, 16, 8, lines=n not in (16, 24, 32, 36, 48, 64, 96))
+                                 if live64 is not None else None),
         "accum_ms_per_block": {"fp32": round(live32[3], 4),
                                "fp64": round(live64[3], 4) if live64 is not None else None},
         "fp32_pipeline_check": check,

@@ -167,9 +167,9 @@ int sep_filter_batch(int ndots, sep_ctx *const *ctxs, const void *const *psi0_de
  * pass A (xy forward), pass B (z pencils with k^2), pass C (xy backward +
  * Newton epilogue) and one accumulation launch (K = 64 Newton vectors,
  * ms targets, 2 nvec real states); kernel_ms[4] = passes A+B+C timed as one
- * span; kernel_ms[5] = the filter loop's fused kernel (pass C with the next
- * step's pass A), 0 where the grid runs the passes separately.  kernel_ms
- * holds 6 values.  No counterpart in the reference (it only prints clock() totals,
+ * span; kernel_ms[5] = one Newton step of the filter loop's split form
+ * (the spectral passes and the kernel that fuses pass C with the next step's
+ * pass A), 0 where the grid has no split form.  kernel_ms holds 6 values.  No counterpart in the reference (it only prints clock() totals,
  * filter/main.c:63-66,91-100). */
 int sep_bench_hpsi(sep_ctx *ctx, int precision, long nvec, long ms, long iters,
                    double *kernel_ms);
@@ -180,7 +180,9 @@ int sep_bench_hpsi(sep_ctx *ctx, int precision, long nvec, long ms, long iters,
  * launch (one host wait per block).  sep_set_timing resets the sums;
  * sep_get_timing returns avg_ms[0..3] = passes A, B, C and the accumulation
  * of a full ring block (where the loop runs pass C fused with the next
- * step's pass A, avg_ms[0] = 0 and avg_ms[2] is the fused kernel), and
+ * step's pass A, avg_ms[0] = 0, avg_ms[1] covers the spectral passes -- the
+ * row + line family's y lines, z pencils and y lines -- and avg_ms[2] the
+ * fused kernel), and
  * samples[0..1] = the number of sampled steps and
  * of sampled full blocks.  No counterpart in the reference (it only prints
  * clock() totals, filter/main.c:63-66,91-100). */
@@ -189,8 +191,8 @@ int sep_set_timing(sep_ctx *ctx, int on);
  * resident random vectors with the plane kernels (when the line length has
  * them) and with the row + strided-line kernels, and keeps the faster family
  * for this context and precision.  ms_plane (-1 when unavailable) and
- * ms_lines (optional) get the measured time of one Newton step's H.psi
- * (plane: pass B + the fused pass C / next pass A where the grid has it).
+ * ms_lines (optional) get the measured time of one Newton step's H.psi in
+ * the filter loop's split form (A + B + C where a family has none).
  * No counterpart in the reference (FFTW plans with FFTW_ESTIMATE,
  * filter/filter.c:22-23).  Uses the
  * context's state and ring buffers as scratch: call it before filtering. */

@@ -1346,8 +1346,8 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
       }
       hipEventElapsedTime(&t, ev[0], ev[3]);
       acc[4] += t;
-      if (fused) {
-        hipEventElapsedTime(&t, ev[6], ev[7]);
+      if (fused) {  // the whole split step: spectral passes + fused kernel
+        hipEventElapsedTime(&t, ev[5], ev[7]);
         acc[5] += t;
       }
     }
@@ -1371,9 +1371,10 @@ extern "C" int sep_bench_hpsi(sep_ctx *c, int precision, long nvec, long ms, lon
 }
 
 // Kernel-family choice per grid and precision: time one Newton step's H.psi
-// with the plane kernels (pass B + the fused pass C / next pass A where the
-// grid has it, else A + B + C) and with the row + strided-line kernels
-// (A + B + C) on nvec resident random vectors (bench_impl); keep the faster.
+// in the filter loop's split form (spectral passes + fused kernel; A + B + C
+// where a family has no split form) with the plane kernels and with the row +
+// strided-line kernels on nvec resident random vectors (bench_impl); keep the
+// faster.
 extern "C" int sep_autotune(sep_ctx *c, int precision, long nvec, double *ms_plane,
                             double *ms_lines) {
   if (!c || nvec < 1 || (precision != SEP_PREC_FP64 && precision != SEP_PREC_FP32))
@@ -1391,9 +1392,8 @@ extern "C" int sep_autotune(sep_ctx *c, int precision, long nvec, double *ms_pla
       c->gd.force_lines[precision] = had;
       return rc;
     }
-    // the filter loop's plane step: pass B + the fused pass C / next pass A
-    // where the grid has it, else passes A + B + C
-    tp = k[5] > 0.0 && !c->gd.no_fuse ? k[1] + k[5] : k[4];
+    // the filter loop's split step (spectral passes + fused kernel)
+    tp = k[5] > 0.0 && !c->gd.no_fuse ? k[5] : k[4];
   }
   c->gd.force_lines[precision] = true;
   const int rc = precision == SEP_PREC_FP64 ? bench_impl<double>(c, nvec, 1, 5, k)
@@ -1402,7 +1402,7 @@ extern "C" int sep_autotune(sep_ctx *c, int precision, long nvec, double *ms_pla
     c->gd.force_lines[precision] = had;
     return rc;
   }
-  tl = k[4];
+  tl = k[5] > 0.0 && !c->gd.no_fuse ? k[5] : k[4];
   c->gd.force_lines[precision] = !(tp > 0.0 && tp <= tl);
   if (ms_plane) *ms_plane = tp;
   if (ms_lines) *ms_lines = tl;

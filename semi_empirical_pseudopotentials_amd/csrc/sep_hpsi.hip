@@ -153,6 +153,79 @@ __global__ void __launch_bounds__(T) k_rows_bwd(const cx<R> *tb, VecView<R> pin,
   if (epi.part) block_sum4<T>(s, epi.part, (long)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
+// The row + line family's counterpart of k_plane_bwd_fwd_w: the inverse x
+// FFT and epilogue of Newton step j (k_rows_bwd) and the forward x FFT of the
+// resulting psi_j row for step j+1 (k_rows_fwd) in one kernel; the row of tb
+// read at the start is rewritten at the end by the same threads.  Saves the
+// forward kernel's read of psi_j.
+template <typename R, int NX, int T>
+__global__ void __launch_bounds__(T) k_rows_bwd_fwd(cx<R> *tb, VecView<R> pin, VecView<R> out,
+                                                    Epi<R> epi, long nrows, long N,
+                                                    const cx<R> *twx) {
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, ROWS = T / G, P = PL::P, PN = lds_line<NX>();
+  __shared__ cx<R> buf[ROWS][PN];
+  __shared__ cx<R> tw[NX];
+  load_tw<R, NX>(tw, twx);
+  __syncthreads();
+  const long vec = blockIdx.y;
+  const int rl = threadIdx.x / G;
+  const int g = threadIdx.x % G;
+  const long row = (long)blockIdx.x * ROWS + rl;
+  const bool act = (rl < ROWS) && (row < nrows);
+  const int rls = rl < ROWS ? rl : 0;
+  TwFor<NX, 1, R> tcb;
+  tcb.init(g, tw);
+  cx<R> v[E];
+  constexpr int r0 = Pass<NX, 1>::radix(0);
+  constexpr int rl_ = Pass<NX, 1>::radix(P - 1);
+  constexpr int rlf = Pass<NX, -1>::radix(P - 1);
+  cx<R> p[E];
+  R V[E];
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / r0; b++)
+#pragma unroll
+      for (int k = 0; k < r0; k++) v[b * r0 + k] = tb[vec * N + row * NX + in_idx<NX, 1, 0>(g, b, k)];
+#pragma unroll
+    for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+      for (int k = 0; k < rl_; k++) {
+        const long gi = row * NX + out_idx<NX, 1, P - 1>(g, b, k);
+        p[b * rl_ + k] = pin.ld(vec, gi);
+        V[b * rl_ + k] = epi.pot[gi];
+      }
+  }
+  auto at = [&](int i) -> cx<R> & { return buf[rls][lds_pad(i)]; };
+  run_passes_c<NX, 1, 0, P>(v, g, act, tcb, at);
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / rl_; b++)
+#pragma unroll
+      for (int k = 0; k < rl_; k++) {
+        const long gi = row * NX + out_idx<NX, 1, P - 1>(g, b, k);
+        v[b * rl_ + k] = epilogue(epi, v[b * rl_ + k], p[b * rl_ + k], V[b * rl_ + k], s);
+        out.st(vec, gi, v[b * rl_ + k]);
+      }
+  }
+  // psi_j's row through the row buffer into the forward input order
+  __syncthreads();
+  write_out<NX, 1, P - 1>(v, g, act, at);
+  __syncthreads();
+  read_in<NX, -1, 0>(v, g, act, at);
+  TwFor<NX, -1, R> tcf;
+  tcf.init(g, tw);
+  run_passes_c<NX, -1, 0, P>(v, g, act, tcf, at);
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < E / rlf; b++)
+#pragma unroll
+      for (int k = 0; k < rlf; k++)
+        tb[vec * N + row * NX + out_idx<NX, -1, P - 1>(g, b, k)] = v[b * rlf + k];
+  }
+}
+
 // =============================================== strided lines (y, fallback)
 // Lines of length N along an axis with element stride `es`; the block takes W
 // consecutive x positions.  Line base for (x, outer) = outer*os + x.
@@ -888,6 +961,9 @@ struct HOps {
                         const cx<R> *tw, hipStream_t st);
   static bool plane_bwd(int nxy, const cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
                         long N, int nz, int nvec, const cx<R> *tw, hipStream_t st, long *nblk);
+  // k_rows_bwd + the next step's k_rows_fwd in one kernel
+  static bool rows_bwd_fwd(int nx, cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
+                           long nrows, long N, int nvec, const cx<R> *tw, hipStream_t st);
   // pass C fused with the next pass A; false where the plane needs chunks
   static bool plane_bwd_fwd(int nxy, cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
                             long N, int nz, int nvec, const cx<R> *tw, hipStream_t st);
@@ -925,6 +1001,24 @@ bool HOps<R>::rows_bwd(int nx, const cx<R> *tb, VecView<R> pin, VecView<R> out, 
     hipLaunchKernelGGL((k_rows_bwd<R, n, T>), grid, dim3(T), 0, st, tb, pin, out, epi, nrows, \
                        N, tw);                                                              \
     return true;                                                                            \
+  }
+    SEP_LINE_SIZES(C)
+#undef C
+  }
+  return false;
+}
+
+template <typename R>
+bool HOps<R>::rows_bwd_fwd(int nx, cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
+                           long nrows, long N, int nvec, const cx<R> *tw, hipStream_t st) {
+  switch (nx) {
+#define C(n)                                                                                 \
+  case n: {                                                                                  \
+    constexpr int T = 256, ROWS = T / FftPlan<n>::G;                                         \
+    dim3 grid((unsigned)((nrows + ROWS - 1) / ROWS), (unsigned)nvec);                        \
+    hipLaunchKernelGGL((k_rows_bwd_fwd<R, n, T>), grid, dim3(T), 0, st, tb, pin, out, epi,    \
+                       nrows, N, tw);                                                        \
+    return true;                                                                             \
   }
     SEP_LINE_SIZES(C)
 #undef C
@@ -1238,13 +1332,20 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
 // Split Newton step: the filter loop runs pass A once on psi_0 (hpsi_fwd),
 // then per step pass B and pass C fused with the next step's pass A
 // (hpsi_step with fwd_next), the spectrum of psi_j staying in tb between
-// steps.  Available where the plane kernels hold a whole plane in LDS and
-// both planes of the vectors exist (the fused kernel keeps the imaginary
-// plane that a real-only view would drop).
+// steps.  Plane family: the fused plane kernel, where the plane kernels hold
+// a whole plane in LDS; row + line family: the y lines of pass A move to the
+// start of the next step and the x rows of pass C and the next pass A run as
+// one kernel.  Both planes of the vectors must exist (the fused kernels keep
+// the imaginary plane that a real-only view would drop).
+static bool split_plane(const GridDev &gd, int prec) {
+  return !gd.force_lines[prec] && plane_supported(gd.nx, gd.ny, prec);
+}
+
 template <typename R>
 bool hpsi_fused_supported(const GridDev &gd) {
   const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
-  if (gd.force_lines[prec] || !plane_supported(gd.nx, gd.ny, prec) || gd.pden > 0.0) return false;
+  if (gd.pden > 0.0) return false;
+  if (!split_plane(gd, prec)) return hop_supported(gd.nx) && hop_supported(gd.ny);
   VecView<R> none{nullptr, nullptr, nullptr, 0};
   Epi<R> epi{nullptr, 0.0, 0.0, nullptr};
   return HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, nullptr, nullptr);
@@ -1252,34 +1353,53 @@ bool hpsi_fused_supported(const GridDev &gd) {
 
 template <typename R>
 int hpsi_fwd(const GridDev &gd, const VecView<R> &in, cx<R> *tb, int nvec, hipStream_t st) {
+  const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
   const cx<R> *twx = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
   if (!hpsi_fused_supported<R>(gd) || !in.im) return SEP_ERR_UNSUPPORTED;
-  if (!HOps<R>::plane_fwd(gd.nx, in, tb, gd.N, gd.nz, nvec, twx, st)) return SEP_ERR_UNSUPPORTED;
+  const bool ok = split_plane(gd, prec)
+                      ? HOps<R>::plane_fwd(gd.nx, in, tb, gd.N, gd.nz, nvec, twx, st)
+                      : HOps<R>::rows_fwd(gd.nx, in, tb, (long)gd.ny * gd.nz, gd.N, nvec, twx, st);
+  if (!ok) return SEP_ERR_UNSUPPORTED;
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 
-// tb holds the spectrum of in (psi_{j-1}); out <- alpha (T + V) in - beta in
-// and, with fwd_next, tb <- the spectrum of out.  Events as hpsi_apply's
-// ev[1..3] (before pass B, after pass B, after the fused kernel); ev[0] is
-// not recorded.
+// tb holds the spectrum of in (psi_{j-1}; for the row + line family its x
+// transform only); out <- alpha (T + V) in - beta in and, with fwd_next, tb
+// <- the same transform of out.  Events as hpsi_apply's ev[1..3]: before the
+// spectral passes (the plane family's pass B; the line family's y lines, z
+// pencils and y lines), after them, after the fused kernel; ev[0] is not
+// recorded.
 template <typename R>
 int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
               const R *pot, int nvec, double alpha, double beta, bool fwd_next, hipStream_t st,
               hipEvent_t *ev) {
   const int nx = gd.nx, ny = gd.ny, nz = gd.nz;
   const long N = gd.N;
+  const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
   const cx<R> *twx = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[0] : gd.tw32[0]);
+  const cx<R> *twy = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[1] : gd.tw32[1]);
   const cx<R> *twz = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[2] : gd.tw32[2]);
   if (!hpsi_fused_supported<R>(gd) || !in.im || !out.im) return SEP_ERR_UNSUPPORTED;
+  const bool plane = split_plane(gd, prec);
   if (ev) hipEventRecord(ev[1], st);
-  const bool cm = plane_is_colmajor(nx);
+  if (!plane && !HOps<R>::lines(ny, -1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st))
+    return SEP_ERR_UNSUPPORTED;
+  const bool cm = plane && plane_is_colmajor(nx);
   if (!HOps<R>::pencil(nz, tb, N, cm ? ny : nx, cm ? nx : ny, nvec, twz, cm ? gd.kyy : gd.kxx,
                        cm ? gd.kxx : gd.kyy, gd.kzz, gd.ekin, 1.0, 0.0, st))
     return SEP_ERR_UNSUPPORTED;
+  if (!plane && !HOps<R>::lines(ny, 1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st))
+    return SEP_ERR_UNSUPPORTED;
   if (ev) hipEventRecord(ev[2], st);
   Epi<R> epi{pot, alpha, beta, nullptr};
-  const bool ok = fwd_next ? HOps<R>::plane_bwd_fwd(nx, tb, in, out, epi, N, nz, nvec, twx, st)
-                           : HOps<R>::plane_bwd(nx, tb, in, out, epi, N, nz, nvec, twx, st, nullptr);
+  bool ok;
+  if (plane)
+    ok = fwd_next ? HOps<R>::plane_bwd_fwd(nx, tb, in, out, epi, N, nz, nvec, twx, st)
+                  : HOps<R>::plane_bwd(nx, tb, in, out, epi, N, nz, nvec, twx, st, nullptr);
+  else
+    ok = fwd_next
+             ? HOps<R>::rows_bwd_fwd(nx, tb, in, out, epi, (long)ny * nz, N, nvec, twx, st)
+             : HOps<R>::rows_bwd(nx, tb, in, out, epi, (long)ny * nz, N, nvec, twx, st, nullptr);
   if (!ok) return SEP_ERR_UNSUPPORTED;
   if (ev) hipEventRecord(ev[3], st);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;

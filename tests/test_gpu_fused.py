@@ -1,7 +1,8 @@
-"""The filter loop's fused kernel (pass C of Newton step j with pass A of
-step j+1, k_plane_bwd_fwd_w) against the same loop with the passes as
-separate kernels (test hook SEP_SEPARATE_PASSES), for odd and even state
-counts, on plane lengths with 256-, 512- and 1024-thread plane kernels.
+"""The filter loop's fused kernels (pass C of Newton step j with pass A of
+step j+1: k_plane_bwd_fwd_w, and k_rows_bwd_fwd for the row + line family)
+against the same loop with the passes as separate kernels (test hook
+SEP_SEPARATE_PASSES), for odd and even state counts, on plane lengths with
+256-, 512- and 1024-thread plane kernels and on the line family.
 FP64 agrees to 1e-13.  FP32 agrees to rounding only: the compiler contracts
 the complex products of the two kernels into FMAs differently, and the
 Newton recurrence carries those single-rounding differences through the
@@ -47,15 +48,16 @@ def _run(sep, d, psi0, nc, ms, prec):
         ctx.close()
 
 
+@pytest.mark.parametrize("family", ["plane", "lines"])
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("n,ns", [(32, 3), (64, 4), (96, 2), (128, 1)])
-def test_fused_equals_separate(sep, monkeypatch, prec, n, ns):
-    if prec == "fp64" and n == 128:
-        pytest.skip("FP64 at 128 runs the row + line kernels (no plane kernel to fuse)")
+def test_fused_equals_separate(sep, monkeypatch, family, prec, n, ns):
+    if family == "lines":  # k_rows_bwd_fwd (FP64 at 128 takes it without the hook)
+        monkeypatch.setenv("SEP_FORCE_LINE_KERNELS", "1")
     P = sep.SEP_PREC_FP32 if prec == "fp32" else sep.SEP_PREC_FP64
     nc, ms = 48, 3
     d, psi0 = _setup(sep, n, 1.2 + n / 64.0, ns, nc, ms)
-    print("case fused", prec, n, ns, flush=True)
+    print("case fused", family, prec, n, ns, flush=True)
     fused, kms = _run(sep, d, psi0, nc, ms, P)
     assert kms[5] > 0.0  # the fused kernel exists for this grid
     monkeypatch.setenv("SEP_SEPARATE_PASSES", "1")
