@@ -254,7 +254,9 @@ int sep_hf_adjoint(sep_ctx *ctx, long npairs, const long *pairs, const double *w
 #define SEP_FP32_SVD_FLOOR 1.0e-7
 /* portho() (filter/ortho.c:11-52) for the FP32 pipeline: Gram matrix of the
  * resident states (FP64 dgemm), its eigen-decomposition on the device, basis
- * A W S^-1 (FP64 dgemm), then two FP64 Cholesky-QR passes.  Directions with
+ * A W S^-1 (FP64 dgemm), then FP64 Cholesky-QR (a second pass only if the
+ * first Gram matrix, scaled to unit diagonal, is not within 0.5 of the
+ * identity in Frobenius norm).  Directions with
  * S_i/S_0 below max(cutoff, SEP_FP32_SVD_FLOOR) are dropped.  The states of a
  * complex64 filter carry a noise floor near 1e-5 of S_0, so the floor drops
  * only exactly dependent directions. */
@@ -264,7 +266,9 @@ int sep_ortho_fp32(sep_ctx *ctx, long m, double cutoff, double *S, long *kept);
 /* One FP64 correction pass on m resident Ritz pairs with values eval:
  * residuals r_i = H psi_i - e_i psi_i, preconditioned t_i = (T + shift)^-1 r_i
  * (T the kinetic energy, applied in reciprocal space), Rayleigh-Ritz on
- * span{psi, t} in FP64, and the m pairs of smallest sigma are kept (eval and
+ * span{psi, t} in FP64 (the generalized problem on its eigen-whitened basis,
+ * which leaves the Ritz states orthonormal to rounding), and the m pairs of
+ * smallest sigma are kept (eval and
  * sige overwritten, ascending in energy).  Returns SEP_ERR_NUMERIC if the
  * enlarged basis holds fewer than m directions. */
 int sep_refine(sep_ctx *ctx, long m, double *eval, double *sige);

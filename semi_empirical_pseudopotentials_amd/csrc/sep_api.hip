@@ -944,6 +944,28 @@ static int sub_gram(sep_ctx *c, const double *A, long m, const double *B, long m
   return SEP_OK;
 }
 
+// Symmetric C = A B^T, m x m (B = A for a Gram matrix, B = H A for a
+// projected Hamiltonian, symmetric in exact arithmetic): only the 512-blocks
+// on and above the block diagonal are multiplied, those below take their
+// mirror, and the whole is scaled.
+static int sub_gram_sym(sep_ctx *c, const double *A, const double *B, long m, double scale,
+                        double *C_d) {
+  RCHK(dense_init(c, 0));
+  const long N = c->gd.N;
+  const double one = 1.0, zero = 0.0;
+  for (long i0 = 0; i0 < m; i0 += 512)
+    for (long j0 = i0; j0 < m; j0 += 512) {
+      const long mi = std::min(512L, m - i0), mj = std::min(512L, m - j0);
+      rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none,
+                                        (rocblas_int)mj, (rocblas_int)mi, (rocblas_int)N, &one,
+                                        B + j0 * N, (rocblas_int)N, A + i0 * N, (rocblas_int)N, &zero,
+                                        C_d + j0 + i0 * m, (rocblas_int)m);
+      if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "Gram: rocblas_dgemm status %d", (int)rs);
+    }
+  return symmetrize_dev(C_d, m, scale, c->st, 512) == SEP_OK ? SEP_OK
+                                                             : fail(c, SEP_ERR_HIP, "symmetrize launch");
+}
+
 // Out[j][g] = sum_i M[i * ldm + j] A[i][g] (mo states out, Out != A):
 // column-major Out (N x mo) = A (N x m) M^T with M column-major (ldm x m).
 static int sub_combine(sep_ctx *c, const double *A, long m, const double *M_d, long ldm, long mo,
@@ -958,20 +980,34 @@ static int sub_combine(sep_ctx *c, const double *A, long m, const double *M_d, l
                                       : fail(c, SEP_ERR_HIP, "rotation: rocblas_dgemm status %d", (int)rs);
 }
 
-// Two Cholesky-QR passes in FP64 on the k resident states: G = A A^T,
-// R = chol(G) and R^-1, A <- R^-T A, all on the device.  Leaves the states
-// orthonormal to FP64 rounding.
-static int cholqr2(sep_ctx *c, long k) {
-  const long N = c->gd.N;
+// Cholesky-QR in FP64 on the k resident states: G = A A^T, R = chol(G),
+// R^-1 and A <- R^-T A, all on the device.  One pass leaves |A A^T - I| at
+// about eps kappa(A)^2, so a second pass follows unless the first Gram
+// matrix, scaled to unit diagonal, lies within 0.5 of the identity in
+// Frobenius norm (then kappa(A)^2 < 3): true of a basis that is already
+// orthonormal up to rounding, as the whitened states of the ortho stage are.
+static int cholqr(sep_ctx *c, long k) {
   RCHK(ensure_states(c, k, 2));
   for (int pass = 0; pass < 2; pass++) {
     RCHK(ensure_scr(c, k * k));
     double *G_d = c->scr;
-    RCHK(sub_gram(c, c->states, k, c->states, k, G_d));
+    RCHK(sub_gram_sym(c, c->states, c->states, k, 1.0, G_d));
+    bool last = pass == 1;
+    if (!last) {
+      std::vector<double> g(k * k);
+      HIPCHK(c, hipMemcpyAsync(g.data(), G_d, sizeof(double) * k * k, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipStreamSynchronize(c->st));
+      double off = 0.0;
+      for (long i = 0; i < k; i++)
+        for (long j = 0; j < k; j++)
+          if (i != j) off += g[i * k + j] * g[i * k + j] / (g[i * k + i] * g[j * k + j]);
+      last = off < 0.25;  // false for NaN (a zero state): the factorisation reports it
+    }
     RCHK(dev_chol_inv(c, k, G_d, "ortho: re-orthogonalisation"));
     RCHK(sub_combine(c, c->states, k, G_d, k, k, c->states2));
     std::swap(c->states, c->states2);
     std::swap(c->cap1, c->cap2);
+    if (last) break;
   }
   return SEP_OK;
 }
@@ -1030,7 +1066,7 @@ extern "C" int sep_ortho(sep_ctx *c, long m, double cutoff, double *S, long *kep
   RCHK(sub_combine(c, c->states, m, M_d, k, k, c->states2));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
-  RCHK(cholqr2(c, k));
+  RCHK(cholqr(c, k));
   *kept = k;
   return normalize_dev(c, c->states, k);  // filter/main.c:112
 }
@@ -1056,16 +1092,18 @@ extern "C" int sep_rayleigh_ritz(sep_ctx *c, long m, double *eval) {
 
 // Orthonormal basis of the span of the m resident states from their FP64
 // Gram matrix: G = W L W^T on the device, directions with sqrt(l_i / l_0) <
-// floor dropped, basis A W_k L_k^-1/2 left in c->states, then polished by two FP64 Cholesky-QR passes.  S gets
+// floor dropped, basis A W_k L_k^-1/2 left in c->states, then (polish)
+// Cholesky-QR in FP64 to orthonormality at rounding level.  Without polish
+// the basis is orthonormal only to the accuracy of G's small eigenpairs, for
+// a caller that solves the generalized problem on it (rr_polished).  S gets
 // sqrt(l) descending.
-static int eig_basis(sep_ctx *c, long m, double floor, double *S, long *kept) {
+static int eig_basis(sep_ctx *c, long m, double floor, double *S, long *kept, bool polish) {
   const long N = c->gd.N;
   RCHK(ensure_states(c, m, 2));
   const double *A = c->states;
   RCHK(ensure_scr(c, 2 * m * m + m));
   double *G_d = c->scr, *M_d = G_d + m * m, *sc_d = M_d + m * m;
-  RCHK(sub_gram(c, A, m, A, m, G_d));
-  RCHK(symmetrize_dev(G_d, m, 1.0, c->st));
+  RCHK(sub_gram_sym(c, A, A, m, 1.0, G_d));
   std::vector<double> w(m);
   RCHK(dev_symeig(c, m, G_d, w.data(), "ortho"));
   std::vector<double> s(m);
@@ -1087,15 +1125,52 @@ static int eig_basis(sep_ctx *c, long m, double floor, double *S, long *kept) {
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
   HIPCHK(c, hipStreamSynchronize(c->st));  // sc outlives the async copy
-  RCHK(cholqr2(c, k));
+  if (polish) RCHK(cholqr(c, k));
   *kept = k;
+  return normalize_dev(c, c->states, k);
+}
+
+// Rayleigh-Ritz on the span of k resident states Y that are orthonormal
+// only approximately (an unpolished eig_basis): the generalized problem
+// P x = e G x with G = Y Y^T and P = Y H Y^T, through G = R^T R and the
+// standard problem R^-T P R^-1 = Q E Q^T.  The states become (R^-1 Q)^T Y --
+// orthonormal to rounding, as after a Cholesky-QR pass followed by
+// sep_rayleigh_ritz, with one combination of the k states instead of two.
+// Energies ascending into eval; states normalized as filter/main.c:122.
+static int rr_polished(sep_ctx *c, long k, double *eval) {
+  RCHK(ensure_states(c, k, 2));
+  RCHK(apply_states(c, c->states, c->states2, k, nullptr, nullptr));  // H y_i
+  RCHK(ensure_scr(c, 3 * k * k));
+  double *U_d = c->scr, *P_d = U_d + k * k, *W_d = P_d + k * k;
+  RCHK(sub_gram_sym(c, c->states, c->states, k, 1.0, U_d));
+  RCHK(sub_gram_sym(c, c->states, c->states2, k, 1.0, P_d));
+  RCHK(dev_chol_inv(c, k, U_d, "Rayleigh-Ritz: Gram factor"));  // U = R^-1, row-major
+  // Column-major views: U_d holds U^T, P_d holds P (symmetric).
+  // W = P U, then P' = U^T W = R^-T P R^-1 over P_d.
+  const double one = 1.0, zero = 0.0;
+  const rocblas_int n = (rocblas_int)k;
+  rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, n, n,
+                                    n, &one, P_d, n, U_d, n, &zero, W_d, n);
+  if (rs == rocblas_status_success)
+    rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, U_d, n,
+                       W_d, n, &zero, P_d, n);
+  if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "Rayleigh-Ritz: rocblas_dgemm status %d", (int)rs);
+  RCHK(symmetrize_dev(P_d, k, 1.0, c->st));
+  RCHK(dev_symeig(c, k, P_d, eval, "Rayleigh-Ritz"));  // Q column-major in P_d
+  // coefficients C = U Q, row-major: as column-major it is C^T = Q^T U^T
+  rs = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one, P_d,
+                     n, U_d, n, &zero, W_d, n);
+  if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "Rayleigh-Ritz: rocblas_dgemm status %d", (int)rs);
+  RCHK(sub_combine(c, c->states, k, W_d, k, k, c->states2));
+  std::swap(c->states, c->states2);
+  std::swap(c->cap1, c->cap2);
   return normalize_dev(c, c->states, k);
 }
 
 extern "C" int sep_ortho_fp32(sep_ctx *c, long m, double cutoff, double *S, long *kept) {
   if (!c || m < 1 || m > c->cap1 || !kept || !(cutoff >= 0.0)) return SEP_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
-  return eig_basis(c, m, cutoff > SEP_FP32_SVD_FLOOR ? cutoff : SEP_FP32_SVD_FLOOR, S, kept);
+  return eig_basis(c, m, cutoff > SEP_FP32_SVD_FLOOR ? cutoff : SEP_FP32_SVD_FLOOR, S, kept, true);
 }
 
 // dst = P src for m real states (FP64), P = 1 / (T + shift) with T the
@@ -1160,9 +1235,9 @@ extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
                            hipMemcpyDeviceToDevice, c->st));
   RCHK(normalize_dev(c, c->states + m * N, m));
   long k = 0;
-  RCHK(eig_basis(c, 2 * m, SEP_FP32_SVD_FLOOR, nullptr, &k));
+  RCHK(eig_basis(c, 2 * m, SEP_FP32_SVD_FLOOR, nullptr, &k, false));
   std::vector<double> ev(k), sg(k);
-  RCHK(sep_rayleigh_ritz(c, k, ev.data()));
+  RCHK(rr_polished(c, k, ev.data()));
   RCHK(sep_sigma(c, k, sg.data()));
   // keep the m pairs of smallest variance, ascending in energy
   std::vector<long> idx(k);
@@ -1171,19 +1246,16 @@ extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
   const long keep = k < m ? k : m;
   idx.resize(keep);
   std::sort(idx.begin(), idx.end());  // Ritz values are ascending in index
-  std::vector<double> P(k * keep, 0.0);
+  // gather the kept states to the front in place: idx ascends with
+  // idx[j] >= j, so no source is overwritten before it is read
   for (long j = 0; j < keep; j++) {
-    P[idx[j] * keep + j] = 1.0;
     eval[j] = ev[idx[j]];
     sige[j] = sg[idx[j]];
+    if (idx[j] != j)
+      HIPCHK(c, hipMemcpyAsync(c->states + j * N, c->states + idx[j] * N, sizeof(double) * N,
+                               hipMemcpyDeviceToDevice, c->st));
   }
   for (long j = keep; j < m; j++) eval[j] = sige[j] = 0.0;
-  RCHK(ensure_scr(c, k * keep));
-  HIPCHK(c, hipMemcpyAsync(c->scr, P.data(), sizeof(double) * k * keep, hipMemcpyHostToDevice, c->st));
-  RCHK(ensure_states(c, keep, 2));
-  RCHK(sub_combine(c, c->states, k, c->scr, keep, keep, c->states2));
-  std::swap(c->states, c->states2);
-  std::swap(c->cap1, c->cap2);
   HIPCHK(c, hipStreamSynchronize(c->st));
   return keep == m ? SEP_OK : fail(c, SEP_ERR_NUMERIC, "refine: basis of %ld < %ld states", keep, m);
 }

@@ -79,7 +79,7 @@ int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStr
 int convert_d2f(const double *in, float *out, long n, hipStream_t st);
 int convert_f2d(const float *in, double *out, long n, hipStream_t st);
 // subspace-stage helpers on n x n row-major matrices (sep_kernels.hip)
-int symmetrize_dev(double *A, long n, double scale, hipStream_t st);
+int symmetrize_dev(double *A, long n, double scale, hipStream_t st, long blk = 0);
 int eigvec_cols_dev(const double *V, long n, long k, const double *sc_d, bool reverse, double *M,
                     hipStream_t st);
 int zero_lower_dev(double *A, long n, hipStream_t st);

@@ -179,7 +179,10 @@ int convert_f2d(const float *in, double *out, long n, hipStream_t st) {
 
 // A <- scale * (A + A^T) / 2 for an n x n row-major matrix; the (i > j)
 // thread of each pair writes both entries, the diagonal is only scaled
-__global__ void k_symmetrize(double *A, long n, double scale) {
+// A <- scale (A + A^T) / 2 for a row-major n x n matrix.  With blk > 0 only
+// the blk x blk blocks on and above the block diagonal hold values: an
+// element of a block below it takes its mirror above, unaveraged.
+__global__ void k_symmetrize(double *A, long n, double scale, long blk) {
   const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long i = (long)blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= n || j >= n || j > i) return;
@@ -187,13 +190,14 @@ __global__ void k_symmetrize(double *A, long n, double scale) {
     A[i * n + i] *= scale;
     return;
   }
-  const double a = 0.5 * (A[i * n + j] + A[j * n + i]) * scale;
+  const double a = (blk > 0 && i / blk != j / blk) ? A[j * n + i] * scale
+                                                   : 0.5 * (A[i * n + j] + A[j * n + i]) * scale;
   A[i * n + j] = a;
   A[j * n + i] = a;
 }
-int symmetrize_dev(double *A, long n, double scale, hipStream_t st) {
-  const dim3 blk(32, 8), grd((unsigned)((n + 31) / 32), (unsigned)((n + 7) / 8));
-  hipLaunchKernelGGL(k_symmetrize, grd, blk, 0, st, A, n, scale);
+int symmetrize_dev(double *A, long n, double scale, hipStream_t st, long blk) {
+  const dim3 blk2(32, 8), grd((unsigned)((n + 31) / 32), (unsigned)((n + 7) / 8));
+  hipLaunchKernelGGL(k_symmetrize, grd, blk2, 0, st, A, n, scale, blk);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 
