@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--ns", type=int, default=16)
     p.add_argument("--ms", type=int, default=32)
     p.add_argument("--nc", type=int, default=4096)
+    p.add_argument("--lanes", type=int, default=0,
+                   help="filter lanes (sep_set_filter_lanes): 0 = automatic, 1 = one stream")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=192,
@@ -254,34 +256,44 @@ def main():
         ctx.close()
         return
     # ------------------------------------------------------------ timed loop
-    # The library samples the three H.psi passes of one Newton step per ring
-    # block, and that block's accumulation, with HIP events on its own stream
-    # while the timed filter runs (sep_set_timing).
+    # The timed passes run as the library runs them by default (filter lanes:
+    # groups of random states on concurrent streams).  One more pass samples
+    # the three H.psi passes of one Newton step per ring block, and that
+    # block's accumulation, with HIP events (sep_set_timing); live timing
+    # keeps the loop on one stream, so those are the kernels' own durations.
+    ctx.set_filter_lanes(args.lanes)
+
     def run(prec):
         for _ in range(args.warmup):
             ctx.filter_dev(psi0_dev.data_ptr(), ns, an, zn, vmin, dE, prec)
         torch.cuda.synchronize()
-        ctx.set_timing(True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             ctx.filter_dev(psi0_dev.data_ptr(), ns, an, zn, vmin, dE, prec)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
+        ctx.set_timing(True)
+        t1 = time.perf_counter()
+        ctx.filter_dev(psi0_dev.data_ptr(), ns, an, zn, vmin, dE, prec)
+        torch.cuda.synchronize()
+        t_one = time.perf_counter() - t1
         live, samples = ctx.get_timing()
         ctx.set_timing(False)
-        return t, live, samples
+        return t, live, samples, t_one
 
-    t32, live32, smp32 = run(sep.SEP_PREC_FP32)
+    t32, live32, smp32, one32 = run(sep.SEP_PREC_FP32)
     hpsi = ns * (nc - 1) * args.steps
     v32 = hpsi / t32
-    log(f"fp32 filter: {t32 / args.steps:.3f} s/step, {v32:.0f} H.psi/s; live kernel ms "
+    log(f"fp32 filter: {t32 / args.steps:.3f} s/step, {v32:.0f} H.psi/s "
+        f"(one stream, timed: {one32:.3f} s); live kernel ms "
         f"A {live32[0]:.4f} B {live32[1]:.4f} C {live32[2]:.4f} accum {live32[3]:.3f} "
         f"(samples {smp32})")
-    v64 = live64 = None
+    v64 = live64 = one64 = None
     if not args.no_fp64:
-        t64, live64, smp64 = run(sep.SEP_PREC_FP64)
+        t64, live64, smp64, one64 = run(sep.SEP_PREC_FP64)
         v64 = hpsi / t64
-        log(f"fp64 filter: {t64 / args.steps:.3f} s/step, {v64:.0f} H.psi/s; live kernel ms "
+        log(f"fp64 filter: {t64 / args.steps:.3f} s/step, {v64:.0f} H.psi/s "
+            f"(one stream, timed: {one64:.3f} s); live kernel ms "
             f"A {live64[0]:.4f} B {live64[1]:.4f} C {live64[2]:.4f} accum {live64[3]:.3f}")
 
     # ------------------------------------- whole pipeline at this workload
@@ -450,8 +462,11 @@ def main():
         "config": {"workload": f"CdSe {args.diameter} nm dot ({d['natom']} atoms), {n}^3 grid, "
                                f"{ns} random states x {ms} targets, Newton length {nc}; "
                                "complex64 filter (FP64 accumulators)",
-                   "grid": n, "states": ns, "targets": ms, "newton_length": nc},
+                   "grid": n, "states": ns, "targets": ms, "newton_length": nc,
+                   "filter_lanes": args.lanes or "automatic"},
         "fp64_value": round(v64, 1) if v64 else None,
+        "one_stream_filter_s": {"fp32": round(one32, 4),
+                                "fp64": round(one64, 4) if one64 is not None else None},
         "fp32_over_fp64": round(v32 / v64, 3) if v64 else None,
         "complex_hpsi_per_s": round(v32 / 2, 1),
         "roofline": roofline,

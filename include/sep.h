@@ -187,6 +187,19 @@ int sep_bench_hpsi(sep_ctx *ctx, int precision, long nvec, long ms, long iters,
  * of sampled full blocks.  No counterpart in the reference (it only prints
  * clock() totals, filter/main.c:63-66,91-100). */
 int sep_set_timing(sep_ctx *ctx, int on);
+/* Largest number of filter lanes. */
+#define SEP_MAX_LANES 4
+/* Filter lanes: sep_filter / sep_filter_dev split the random states into
+ * `lanes` groups of whole complex vectors (two states each) that run their
+ * Newton steps and accumulations on streams of their own, so that one
+ * group's kernels overlap another's -- the device counterpart of the
+ * reference's OpenMP loop over random states (filter/main.c:94).  Results
+ * are bitwise those of one lane.  0 (default) = automatic: 2 lanes from 4
+ * complex vectors on; 1 = off; at most SEP_MAX_LANES and one per complex
+ * vector.  Lanes apply to the split-step loop (sep_autotune's families on
+ * grids with a split form) and are off while live timing is on.  No
+ * counterpart in the reference's function interface. */
+int sep_set_filter_lanes(sep_ctx *ctx, int lanes);
 /* Autotuner: times the H.psi passes of this grid in `precision` on nvec
  * resident random vectors with the plane kernels (when the line length has
  * them) and with the row + strided-line kernels, and keeps the faster family

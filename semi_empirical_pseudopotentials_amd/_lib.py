@@ -91,6 +91,7 @@ SIGNATURES = {
     "sep_filter_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D, _I]),
     "sep_bench_hpsi": (_I, [_V, _I, _L, _L, _L, _dp]),
     "sep_set_timing": (_I, [_V, _I]),
+    "sep_set_filter_lanes": (_I, [_V, _I]),
     "sep_autotune": (_I, [_V, _I, _L, _dp, _dp]),
     "sep_filter_batch": (_I, [_I, _V, _V, _lp, _V, _V, _lp, _lp, _dp, _dp, _I, _I]),
     "sep_get_timing": (_I, [_V, _dp, ctypes.POINTER(ctypes.c_long)]),
@@ -208,6 +209,7 @@ def fft_length_supported(n):
     return bool(lib().sep_fft_length_supported(n))
 
 
+SEP_MAX_LANES = 4
 SEP_BATCH_INTERLEAVE = 0
 SEP_BATCH_GRAPH = 1
 
@@ -381,6 +383,11 @@ class Context:
     def set_timing(self, on):
         """Sample kernel durations inside the filter loop (see sep_set_timing)."""
         self._chk(lib().sep_set_timing(self._h, 1 if on else 0), "sep_set_timing")
+
+    def set_filter_lanes(self, lanes):
+        """Concurrent groups of random states in the filter (sep_set_filter_lanes):
+        0 = automatic, 1 = off, up to SEP_MAX_LANES."""
+        self._chk(lib().sep_set_filter_lanes(self._h, int(lanes)), "sep_set_filter_lanes")
 
     def get_timing(self):
         """(avg ms of passes A, B, C, avg ms of one full-block accumulation), samples."""
