@@ -389,10 +389,14 @@ def main():
     kname = {"A": "k_plane_fwd_w<float,128>", "B": "k_pencil<float,128>",
              "C": "k_plane_bwd_w<float,128>", "CA": "k_plane_bwd_fwd_w<float,128>"}[dom]
     traffic = measured_traffic(kname, n, nvec)
+    probe = round(ctx.bench_copy(4 << 30, 5), 1)  # plain 16 B/lane copy, best of 4 grids
     roofline = {"bound": "hbm", "kernel": kname, "achieved": r32[dom]["GBs"],
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": r32[dom]["frac"],
                 "traffic": traffic, "algorithmic_bytes": r32[dom]["bytes"],
-                "hpsi_passes_frac": r32["step"]["frac"]}
+                "hpsi_passes_frac": r32["step"]["frac"],
+                "copy_probe_GBs": probe,
+                "frac_of_copy_probe": round(r32[dom]["GBs"] / probe, 4),
+                "hpsi_passes_frac_of_copy_probe": round(r32["step"]["GBs"] / probe, 4)}
 
     # --------------------------------------- Hellmann-Feynman adjoint kernel
     # one backward of the differentiable path: the ns*ms filtered states of the

@@ -173,6 +173,12 @@ int sep_filter_batch(int ndots, sep_ctx *const *ctxs, const void *const *psi0_de
  * filter/main.c:63-66,91-100). */
 int sep_bench_hpsi(sep_ctx *ctx, int precision, long nvec, long ms, long iters,
                    double *kernel_ms);
+/* HBM bandwidth probe: *gbs = the best rate (GB/s, bytes read + bytes
+ * written) of a plain copy of nbytes (>= 1 MiB) between two device buffers,
+ * 16 bytes per lane, over iters timed launches at each of four grid sizes --
+ * the achievable bandwidth beside the nominal 8 TB/s.  No counterpart in the
+ * reference. */
+int sep_bench_copy(sep_ctx *ctx, long nbytes, long iters, double *gbs);
 
 /* Live kernel timing of the filter loop.  While on, sep_filter/sep_filter_dev
  * record HIP events on the context's stream around the three H.psi passes of

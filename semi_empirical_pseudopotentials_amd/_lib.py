@@ -90,6 +90,7 @@ SIGNATURES = {
     "sep_refine": (_I, [_V, _L, _dp, _dp]),
     "sep_filter_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D, _I]),
     "sep_bench_hpsi": (_I, [_V, _I, _L, _L, _L, _dp]),
+    "sep_bench_copy": (_I, [_V, _L, _L, _dp]),
     "sep_set_timing": (_I, [_V, _I]),
     "sep_set_filter_lanes": (_I, [_V, _I]),
     "sep_autotune": (_I, [_V, _I, _L, _dp, _dp]),
@@ -395,6 +396,12 @@ class Context:
         n = (ctypes.c_long * 2)()
         self._chk(lib().sep_get_timing(self._h, _p(out), n), "sep_get_timing")
         return out, (n[0], n[1])
+
+    def bench_copy(self, nbytes=4 << 30, iters=5):
+        """Achievable HBM bandwidth, GB/s read + written (sep_bench_copy)."""
+        out = np.zeros(1)
+        self._chk(lib().sep_bench_copy(self._h, int(nbytes), int(iters), _p(out)), "sep_bench_copy")
+        return float(out[0])
 
     def gram_dev(self, A_ptr, m, B_ptr, mb, precision=SEP_PREC_FP64):
         C = np.empty((m, mb))

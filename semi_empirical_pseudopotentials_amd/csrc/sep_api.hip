@@ -1505,6 +1505,73 @@ static int bench_impl(sep_ctx *c, long nvec, long ms, long iters, double *kms) {
   return SEP_OK;
 }
 
+// Plain copy of n 16-byte words (the HBM bandwidth probe): four loads in
+// flight per lane, then four stores; the unrolled loop runs only while all
+// four indices are below n, the tail one word at a time.  NT: nontemporal
+// (streaming) loads and stores.
+typedef float word16 __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ word16 ld16(const word16 *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st16(word16 v, word16 *p) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+template <bool NT>
+__global__ void __launch_bounds__(256) k_copy16(const word16 *__restrict__ a,
+                                                word16 *__restrict__ b, long n) {
+  const long stride = (long)gridDim.x * 256;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const word16 x0 = ld16<NT>(a + i), x1 = ld16<NT>(a + i + stride);
+    const word16 x2 = ld16<NT>(a + i + 2 * stride), x3 = ld16<NT>(a + i + 3 * stride);
+    st16<NT>(x0, b + i);
+    st16<NT>(x1, b + i + stride);
+    st16<NT>(x2, b + i + 2 * stride);
+    st16<NT>(x3, b + i + 3 * stride);
+  }
+  for (; i < n; i += stride) st16<NT>(ld16<NT>(a + i), b + i);
+}
+
+extern "C" int sep_bench_copy(sep_ctx *c, long nbytes, long iters, double *gbs) {
+  if (!c || !gbs || nbytes < (1L << 20) || iters < 1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long n = nbytes / 16;
+  DevBuf a, b;
+  RCHK(dev_alloc(c, a, 16 * (size_t)n));
+  RCHK(dev_alloc(c, b, 16 * (size_t)n));
+  RCHK(zero_dev(a.p, 16 * (size_t)n, c->st));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) hipEventDestroy(e0);
+    return fail(c, SEP_ERR_HIP, "copy probe: event creation failed");
+  }
+  double best = 0.0;
+  int rc = SEP_OK;
+  for (int nt = 0; nt < 2; nt++)
+  for (int blocks : {1024, 2048, 4096, 8192})
+    for (long it = 0; it <= iters && rc == SEP_OK; it++) {  // it 0: warm-up
+      hipEventRecord(e0, c->st);
+      hipLaunchKernelGGL(nt ? k_copy16<true> : k_copy16<false>, dim3(blocks), dim3(256), 0,
+                         c->st, static_cast<const word16 *>(a.p), static_cast<word16 *>(b.p), n);
+      hipEventRecord(e1, c->st);
+      float ms = 0.0f;
+      if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(c, SEP_ERR_HIP, "copy probe launch failed");
+      } else if (it > 0 && ms > 0.0f) {
+        best = std::max(best, 2.0 * 16.0 * (double)n / (ms * 1e-3) / 1e9);
+      }
+    }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *gbs = best;
+  return rc;
+}
+
 extern "C" int sep_bench_hpsi(sep_ctx *c, int precision, long nvec, long ms, long iters,
                               double *kernel_ms) {
   if (!c || nvec < 1 || ms < 1 || iters < 1 || !kernel_ms) return SEP_ERR_ARG;

@@ -1,7 +1,8 @@
 """Filter lanes (sep_set_filter_lanes): the random states split into groups
 of whole complex vectors that step on streams of their own give bitwise the
 filtered states of one lane -- odd and even state counts, uneven groups,
-both precisions, both kernel families, two ring blocks (nc > 512)."""
+both precisions, both kernel families, two ring blocks (nc > 512).  Also the
+HBM copy probe (sep_bench_copy) that the bench reports beside the peak."""
 import numpy as np
 import pytest
 
@@ -55,5 +56,19 @@ def test_lanes_argument_checks(sep):
                 ctx.set_filter_lanes(bad)
         ctx.set_filter_lanes(sep.SEP_MAX_LANES)
         ctx.set_filter_lanes(0)
+    finally:
+        ctx.close()
+
+
+def test_copy_probe(sep):
+    """sep_bench_copy: a plain copy's rate is positive and below the nominal
+    8 TB/s peak; sizes under 1 MiB are refused."""
+    ctx = sep.Context(16, 16, 16, 0.5, 0.5, 0.5)
+    try:
+        gbs = ctx.bench_copy(1 << 30, 2)
+        print("copy probe GB/s", gbs, flush=True)
+        assert 100.0 < gbs < 8000.0
+        with pytest.raises(sep.SepError):
+            ctx.bench_copy(1024, 1)
     finally:
         ctx.close()
