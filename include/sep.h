@@ -228,8 +228,8 @@ int sep_normalize_all(sep_ctx *ctx, long m);
 int sep_energy_all(sep_ctx *ctx, long m, double *ene);
 
 /* ---------------------------------------------------------------- subspace */
-/* Orthonormalises the m resident states (thin SVD, shifted Cholesky-QR +
- * one-sided Jacobi on R), keeps the columns with S_i/S_0 >= cutoff (1e-10 in
+/* Orthonormalises the m resident states (thin SVD: double-double Gram,
+ * shifted double-double Cholesky R, SVD of R on the device), keeps the columns with S_i/S_0 >= cutoff (1e-10 in
  * the reference), then normalize_all.  Replaces portho(), filter/ortho.c:11-52
  * (+ normalize_all at filter/main.c:112).  S (optional, m) gets the singular
  * values in descending order, *kept the new m. */

@@ -972,6 +972,35 @@ static int dev_symeig(sep_ctx *c, long n, double *A_d, double *w, const char *wh
   return SEP_OK;
 }
 
+// Singular values (descending, into S) and right singular vectors of the
+// row-major m x m host matrix R on the device (rocSOLVER dgesvd):
+// V[i * m + j] = component i of vector j.  The column-major view of R is
+// R^T = V S U^T, so V is the left factor rocSOLVER returns for that view.
+// Uses c->scr[0 .. 2 m^2).
+static int dev_svd_right(sep_ctx *c, long m, const double *R, double *S, double *V) {
+  RCHK(dense_init(c, m));
+  RCHK(ensure_scr(c, 2 * (size_t)m * m));
+  double *A_d = c->scr, *U_d = A_d + m * m;
+  double *S_d = reinterpret_cast<double *>(c->dws), *E_d = S_d + m;
+  rocblas_int *info_d = reinterpret_cast<rocblas_int *>(E_d + m);
+  HIPCHK(c, hipMemcpyAsync(A_d, R, sizeof(double) * m * m, hipMemcpyHostToDevice, c->st));
+  const rocblas_int n = (rocblas_int)m;
+  rocblas_status rs = rocsolver_dgesvd(c->blas, rocblas_svect_singular, rocblas_svect_none, n, n,
+                                       A_d, n, S_d, U_d, n, nullptr, 1, E_d, rocblas_outofplace,
+                                       info_d);
+  if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "ortho: rocsolver_dgesvd status %d", (int)rs);
+  std::vector<double> u((size_t)m * m);
+  rocblas_int info = 0;
+  HIPCHK(c, hipMemcpyAsync(S, S_d, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(u.data(), U_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (info != 0) return fail(c, SEP_ERR_NUMERIC, "ortho: SVD did not converge (%d)", (int)info);
+  for (long j = 0; j < m; j++)
+    for (long i = 0; i < m; i++) V[i * m + j] = u[i + j * m];
+  return SEP_OK;
+}
+
 // G_d (symmetric positive definite, row-major) <- R^-1 with R^T R = G, R
 // upper triangular, zeros below the diagonal.  Column-major lower Cholesky
 // L L^T = G is the row-major upper R = L^T; its column-major lower inverse
@@ -1114,9 +1143,9 @@ extern "C" int sep_ortho(sep_ctx *c, long m, double cutoff, double *S, long *kep
     st = sep_la_chol_dd(m, ghi.data(), glo.data(), shift, R.data());
   }
   if (st) return fail(c, SEP_ERR_NUMERIC, "ortho: Cholesky pivot %d not positive", st);
-  // 3. SVD of R: A = (A V S^-1) S V^T
-  std::vector<double> U(m * m), V(m * m), s(m);
-  sep_la_svd_jacobi(m, R.data(), U.data(), s.data(), V.data());
+  // 3. SVD of R (device): A = (A V S^-1) S V^T
+  std::vector<double> V(m * m), s(m);
+  RCHK(dev_svd_right(c, m, R.data(), s.data(), V.data()));
   for (long i = 0; i < m; i++) {
     double v = s[i] * s[i] - shift;
     s[i] = v > 0.0 ? sqrt(v) : 0.0;

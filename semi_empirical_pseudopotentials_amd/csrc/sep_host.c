@@ -170,89 +170,6 @@ int sep_newton_coeffs(long nc, long ms, const double *zn, const double *el, doub
 }
 
 /* =========================================================== dense FP64 */
-void sep_la_svd_jacobi(long n, const double *R, double *U, double *S, double *V) {
-  /* columns of B (= R) orthogonalised by plane rotations in a round-robin
-   * order whose pairs within a round are disjoint (parallel). */
-  const long np = n + (n & 1);
-  double *B = (double *)calloc(np * n, sizeof(double)); /* column c at B + c*n */
-  for (long c = 0; c < n; c++)
-    for (long r = 0; r < n; r++) B[c * n + r] = R[r * n + c];
-  double *W = (double *)calloc(np * n, sizeof(double)); /* right vectors, column c at W + c*n */
-  for (long c = 0; c < n; c++) W[c * n + c] = 1.0;
-  long *pos = (long *)malloc(sizeof(long) * np);
-  for (long i = 0; i < np; i++) pos[i] = i;
-  for (int sweep = 0; sweep < 80; sweep++) {
-    int changed = 0;
-    for (long round = 0; round < np - 1; round++) {
-#pragma omp parallel for schedule(dynamic, 4) reduction(| : changed)
-      for (long k = 0; k < np / 2; k++) {
-        long p = pos[k], q = pos[np - 1 - k];
-        if (p >= n || q >= n) continue;
-        if (p > q) {
-          long t = p;
-          p = q;
-          q = t;
-        }
-        double *bp = B + p * n, *bq = B + q * n;
-        double a = 0.0, b = 0.0, g = 0.0;
-        for (long r = 0; r < n; r++) {
-          a += bp[r] * bp[r];
-          b += bq[r] * bq[r];
-          g += bp[r] * bq[r];
-        }
-        if (g == 0.0 || a == 0.0 || b == 0.0) continue;
-        if (fabs(g) <= 1e-15 * sqrt(a) * sqrt(b)) continue;
-        changed = 1;
-        double zeta = (b - a) / (2.0 * g);
-        double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-        double *wp = W + p * n, *wq = W + q * n;
-        for (long r = 0; r < n; r++) {
-          double xp = bp[r], xq = bq[r];
-          bp[r] = c * xp - s * xq;
-          bq[r] = s * xp + c * xq;
-          double yp = wp[r], yq = wq[r];
-          wp[r] = c * yp - s * yq;
-          wq[r] = s * yp + c * yq;
-        }
-      }
-      /* rotate positions 1..np-1 */
-      long last = pos[np - 1];
-      for (long i = np - 1; i > 1; i--) pos[i] = pos[i - 1];
-      pos[1] = last;
-    }
-    if (!changed) break;
-  }
-  double *sv = (double *)malloc(sizeof(double) * n);
-  long *idx = (long *)malloc(sizeof(long) * n);
-  for (long c = 0; c < n; c++) {
-    double s = 0.0;
-    for (long r = 0; r < n; r++) s += B[c * n + r] * B[c * n + r];
-    sv[c] = sqrt(s);
-    idx[c] = c;
-  }
-  for (long i = 1; i < n; i++) {
-    long x = idx[i], j = i;
-    while (j > 0 && sv[idx[j - 1]] < sv[x]) {
-      idx[j] = idx[j - 1];
-      j--;
-    }
-    idx[j] = x;
-  }
-  for (long k = 0; k < n; k++) {
-    long c = idx[k];
-    S[k] = sv[c];
-    for (long r = 0; r < n; r++) U[r * n + k] = sv[c] > 0.0 ? B[c * n + r] / sv[c] : 0.0;
-    if (V)
-      for (long r = 0; r < n; r++) V[r * n + k] = W[c * n + r];
-  }
-  free(W);
-  free(B);
-  free(pos);
-  free(sv);
-  free(idx);
-}
-
 /* ------------------------------------------- double-double Cholesky */
 typedef struct {
   double hi, lo;
