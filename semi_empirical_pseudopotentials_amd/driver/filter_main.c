@@ -11,6 +11,9 @@
  *   --precision P      fp64 (default) or fp32: complex64 filter, FP64-Gram ortho,
  *                      FP64 Rayleigh-Ritz and one FP64 preconditioned correction pass
  *   --device N         HIP device ordinal
+ *   --lanes L          filter lanes (groups of random states on concurrent
+ *                      streams): 0 = automatic (default), 1 = one stream, up
+ *                      to SEP_MAX_LANES; results do not depend on it
  *   --psi-filt         also write psi-filt-0-<jns>.dat (filter/filter.c:44-48)
  *   --resume-filt      skip the random states and the filter: read the filtered
  *                      states from psi-filt-0-<jns>.dat of an earlier --psi-filt
@@ -83,7 +86,7 @@ static void write_func(const double *an0, const double *zn, long nc, double dE, 
 
 int main(int argc, char **argv) {
   long seed_opt = 0;
-  int have_seed_opt = 0, prec = SEP_PREC_FP64, device = 0, psi_filt = 0, resume = 0;
+  int have_seed_opt = 0, prec = SEP_PREC_FP64, device = 0, psi_filt = 0, resume = 0, lanes = 0;
   for (int i = 1; i < argc; i++) {
     if (!strcmp(argv[i], "--dir") && i + 1 < argc)
       g_dir = argv[++i];
@@ -100,7 +103,12 @@ int main(int argc, char **argv) {
         die("--precision must be fp64 or fp32");
     } else if (!strcmp(argv[i], "--device") && i + 1 < argc)
       device = atoi(argv[++i]);
-    else if (!strcmp(argv[i], "--psi-filt"))
+    else if (!strcmp(argv[i], "--lanes") && i + 1 < argc) {
+      char *end = NULL;
+      const long v = strtol(argv[++i], &end, 10);
+      if (!end || *end || v < 0 || v > SEP_MAX_LANES) die("--lanes must be an integer in 0..4");
+      lanes = (int)v;
+    } else if (!strcmp(argv[i], "--psi-filt"))
       psi_filt = 1;
     else if (!strcmp(argv[i], "--resume-filt"))
       resume = 1;
@@ -156,6 +164,7 @@ int main(int argc, char **argv) {
     }                                                                        \
   } while (0)
 
+  CHECK(sep_set_filter_lanes(ctx, lanes));
   /* V(r) on the device (filter/init.c:157-199) */
   int *ti = (int *)malloc(sizeof(int) * d.natom);
   for (long i = 0; i < d.natom; i++) ti[i] = d.type[i];

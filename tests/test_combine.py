@@ -113,3 +113,15 @@ def test_couplings_driver_rejects_bad_options(tmp_path):
     r = subprocess.run([COUPLINGS_BIN, "--dir", str(tmp_path)], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1
+
+
+def test_filter_driver_rejects_bad_lanes(tmp_path):
+    """bin/filter refuses a --lanes value outside 0..SEP_MAX_LANES before
+    reading the deck or touching a device."""
+    import subprocess
+    from semi_empirical_pseudopotentials_amd._lib import FILTER_BIN
+    for bad in ("9", "-1", "two"):
+        r = subprocess.run([FILTER_BIN, "--dir", str(tmp_path), "--lanes", bad],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1
+        assert "--lanes must be" in open(tmp_path / "error").read()

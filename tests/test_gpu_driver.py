@@ -77,3 +77,19 @@ def test_driver_resume_from_filtered_states(sep, small_deck, tmp_path):
                        text=True, timeout=600)
     assert r.returncode == 1
     assert "psi-filt" in open(os.path.join(again, "error")).read()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_driver_lanes_same_bytes(sep, small_deck, tmp_path, prec):
+    """--lanes 2 (two groups of random states on concurrent streams) writes
+    eval.dat and psi.dat byte for byte as --lanes 1."""
+    print("case driver lanes", prec, flush=True)
+    outs = []
+    for lanes in ("1", "2"):
+        out = str(tmp_path / f"l{lanes}")
+        _run_filter(small_deck, out, "--seed", "-1001", "--precision", prec, "--lanes", lanes)
+        outs.append(out)
+    for f in ("eval.dat", "psi.dat"):
+        a = open(os.path.join(outs[0], f), "rb").read()
+        b = open(os.path.join(outs[1], f), "rb").read()
+        assert len(a) > 0 and a == b, f
