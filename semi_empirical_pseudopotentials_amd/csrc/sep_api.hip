@@ -1128,6 +1128,11 @@ extern "C" int sep_ortho(sep_ctx *c, long m, double cutoff, double *S, long *kep
   HIPCHK(c, hipMemcpyAsync(ghi.data(), ghi_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipMemcpyAsync(glo.data(), glo_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  for (long i = 1; i < m; i++)  // the lower triangle from the upper one
+    for (long j = 0; j < i; j++) {
+      ghi[i * m + j] = ghi[j * m + i];
+      glo[i * m + j] = glo[j * m + i];
+    }
   // 2. R = chol(G) in double-double; a shift at the double-double rounding
   //    level keeps exactly dependent states factorable
   double tr = 0.0;

@@ -490,6 +490,9 @@ __global__ void __launch_bounds__(256) k_gram_dd(const double *__restrict__ A, l
                                                  double *__restrict__ wlo) {
   constexpr int KT = 128;
   __shared__ double Ai[16][KT + 1], Aj[16][KT + 1];
+  // tiles strictly below the diagonal are left to the caller, who mirrors
+  // the upper ones (G is symmetric)
+  if (blockIdx.y < blockIdx.x) return;
   const long i0 = (long)blockIdx.x * 16, j0 = (long)blockIdx.y * 16;
   const long g0 = (long)blockIdx.z * kc;
   long g1 = g0 + kc;

@@ -110,7 +110,8 @@ template <typename R>
 int combine_dev(const R *A, long m, const double *M_d, long ldm, long mo, long N, R *Out,
                 hipStream_t st);
 long gram_work_len(long m, long mb, long N);
-// G = A A^T as double-double (hi + lo), m x m row-major
+// G = A A^T as double-double (hi + lo), m x m row-major: the upper triangle
+// (16 x 16 tiles on and above the diagonal); entries below it are undefined
 int gram_dd_dev(const double *A, long m, long N, double *hi_d, double *lo_d, double *work,
                 long work_len, hipStream_t st);
 long gram_dd_work_len(long m, long N);
