@@ -773,8 +773,9 @@ static int filter_finish(FilterRun<R> &f, int rc) {
 // live timing (its events sample one stream), at most one lane per complex
 // vector.  Automatic: two lanes from 4 complex vectors (8 random states) on
 // -- measured at 128^3 (16 states, nc 4096): complex64 0.864 -> 0.813 s,
-// complex128 2.078 -> 2.005 s; four lanes 0.854 s.  Streams and events are
-// created on first use, at default priority.
+// complex128 2.078 -> 2.005 s; four lanes 0.854 s.  The second lane's stream
+// is created by sep_open, further ones and the events on first use, all at
+// default priority.
 template <typename R>
 static int filter_lanes(sep_ctx *c, const FilterRun<R> &f) {
   if (!f.fused || c->timing || f.nc <= 1) return 1;
