@@ -482,7 +482,8 @@ class Context:
         return dR, ds, dtab
 
     def ortho_fp32(self, m, cutoff=1.0e-10):
-        """ortho with the Gram matrix and basis on FP32 MFMA (sep_ortho_fp32)."""
+        """ortho of the FP32 pipeline (sep_ortho_fp32): FP64 Gram eigenbasis with the
+        SEP_FP32_SVD_FLOOR floor, polished by Cholesky-QR."""
         S = np.empty(m)
         kept = np.zeros(1, dtype=np.int64)
         rc = lib().sep_ortho_fp32(self._h, m, cutoff, _p(S), kept.ctypes.data_as(_lp))

@@ -174,3 +174,35 @@ def test_refine_vs_numpy(sep, oracle, small_deck):
     print("e", np.abs(e_got - e_ref).max(), "sigma", np.abs(s_got - s_ref).max(), flush=True)
     assert np.abs(e_got - e_ref).max() < 1e-9
     assert np.abs(s_got - s_ref).max() < 1e-7
+
+
+def test_ortho_fp32_ill_conditioned(sep):
+    """sep_ortho_fp32 on 64 states of which 6 are nearly dependent: all 64
+    kept, the basis orthonormal to 1e-12 and spanning the input, the
+    singular values those of numpy's SVD to the
+    FP64 Gram's resolution: eigenvalue errors of about eps sqrt(N) S_0^2
+    (N = 32768 terms per entry), i.e. 1e-14 S_0^2 / S_i in S_i (7 % of the
+    smallest here, whose ratios to S_0 are 4e-7 .. 1.3e-6)."""
+    n, m = 32, 64
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((m, n ** 3))
+    for r, eps in zip(range(58, 64), (1e-6, 1.5e-6, 2e-6, 2.5e-6, 3e-6, 1e-6)):
+        A[r] = A[r - 58] + 0.5 * A[r - 57] + eps * rng.standard_normal(n ** 3)
+    S_ref = np.linalg.svd(A, compute_uv=False)
+    print("case ortho_fp32 ill-conditioned", S_ref[-1] / S_ref[0], flush=True)
+    ctx = sep.Context(n, n, n, 0.5, 0.5, 0.5)
+    try:
+        ctx.states_upload(A)
+        k, S = ctx.ortho_fp32(m)
+        Q = ctx.states_download(k)
+        dv = 0.125
+    finally:
+        ctx.close()
+    assert k == m
+    assert np.abs(Q @ Q.T * dv - np.eye(k)).max() < 1e-12
+    # the span: A's rows are reproduced by their projection on Q
+    P = (A @ Q.T) * dv
+    assert np.abs(P @ Q - A).max() < 1e-8 * np.abs(A).max()
+    tol = 1e-14 * S_ref[0] ** 2 / S_ref
+    print("S max rel-to-tol", (np.abs(S - S_ref) / tol).max(), flush=True)
+    assert np.all(np.abs(S - S_ref) <= tol + 1e-13 * S_ref[0])
