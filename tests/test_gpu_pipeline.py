@@ -56,11 +56,15 @@ def _random_states(oracle, ns, N, dv, seed=-1001):
     return np.array([r.init_psi(idc, N, dv) for _ in range(ns)])
 
 
-@pytest.mark.parametrize("ns", [3, 4])
-def test_filter_fp64(sep, oracle, setup, ns):
-    """Filtered states (normalised) match the oracle's to 1e-9 relative."""
+@pytest.mark.parametrize("ns,depth", [(3, None), (4, None), (3, "32"), (4, "40")])
+def test_filter_fp64(sep, oracle, setup, monkeypatch, ns, depth):
+    """Filtered states (normalised) match the oracle's to 1e-9 relative; with
+    the ring depth bounded (test hook SEP_RING_DEPTH) the 95 Newton vectors
+    are accumulated over 3 ring blocks, the last one partial."""
+    if depth:
+        monkeypatch.setenv("SEP_RING_DEPTH", depth)
     d, grid, V, ksq = setup
-    print("case filter_fp64 ns", ns, flush=True)
+    print("case filter_fp64 ns", ns, "ring depth", depth, flush=True)
     N, nc, ms = V.size, 96, 4
     dE, zn, el, an = _coeffs(oracle, d, V, ksq, grid, nc, ms)
     psi0 = _random_states(oracle, ns, N, d["dv"])
@@ -78,14 +82,18 @@ def test_filter_fp64(sep, oracle, setup, ns):
     assert np.abs(ene - ene_ref).max() < 1e-10
 
 
-@pytest.mark.parametrize("nc,ms", [(96, 4), (37, 5), (13, 33)])
-def test_filter_fp32(sep, oracle, setup, nc, ms):
+@pytest.mark.parametrize("nc,ms,depth", [(96, 4, None), (37, 5, None), (13, 33, None),
+                                         (96, 4, "32"), (96, 5, "24")])
+def test_filter_fp32(sep, oracle, setup, monkeypatch, nc, ms, depth):
     """complex64 recurrence with FP64 accumulators: close to the FP64 oracle.
     Newton lengths that are not a multiple of 16 (the packed accumulation's
-    partial last row group) and more than 32 targets (a second target group
-    of one) included."""
+    partial last row group), more than 32 targets (a second target group
+    of one) and bounded ring depths (several ring blocks; 24 is not a
+    multiple of 16, so the general accumulation kernel runs) included."""
+    if depth:
+        monkeypatch.setenv("SEP_RING_DEPTH", depth)
     d, grid, V, ksq = setup
-    print("case filter_fp32", nc, ms, flush=True)
+    print("case filter_fp32", nc, ms, "ring depth", depth, flush=True)
     N, ns = V.size, 3
     dE, zn, el, an = _coeffs(oracle, d, V, ksq, grid, nc, ms)
     psi0 = _random_states(oracle, ns, N, d["dv"])
