@@ -639,11 +639,19 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
   const long kmax = kenv && atol(kenv) >= 1 && atol(kenv) <= 1024 ? atol(kenv) : 1024;
   long K = nc < kmax ? nc : kmax;
   while (K > 8 && (double)K * S2 * N * sizeof(R) > cap) K /= 2;
-  f.K = K;
   // ring layout [state][Newton vector][point]: row (s, k) at s * K * N + k * N,
-  // so the accumulation walks each state's K rows within K * N elements
+  // so the accumulation walks each state's K rows within K * N elements;
+  // where the allocation fails (memory the free-memory figure promised went
+  // elsewhere meanwhile) the ring is halved until it fits
+  int rc_ring = ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * S2 * N);
+  while (rc_ring != SEP_OK && K > 8) {
+    (void)hipGetLastError();  // the failed allocation must not be reported by a later launch
+    K /= 2;
+    rc_ring = ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * S2 * N);
+  }
+  RCHK(rc_ring);
+  f.K = K;
   f.slot = N;
-  RCHK(ensure(c, &c->ring, &c->ring_bytes, sizeof(R) * (size_t)K * S2 * N));
   RCHK(ensure(c, &c->tb, &c->tb_bytes, sizeof(cx<R>) * (size_t)f.nvec * N));
   f.ring = reinterpret_cast<R *>(c->ring);
   RCHK(ensure_states(c, ns * ms, 1));
