@@ -193,6 +193,11 @@ int sep_bench_copy(sep_ctx *ctx, long nbytes, long iters, double *gbs);
  * of sampled full blocks.  No counterpart in the reference (it only prints
  * clock() totals, filter/main.c:63-66,91-100). */
 int sep_set_timing(sep_ctx *ctx, int on);
+/* Frees the filter's ring of Newton vectors (up to 160 GB; the context
+ * otherwise keeps it for the next filter until sep_close).  The resident
+ * states and everything else stay; the next filter allocates a ring again.
+ * No counterpart in the reference, which frees nothing before it exits. */
+int sep_trim(sep_ctx *ctx);
 /* Largest number of filter lanes. */
 #define SEP_MAX_LANES 4
 /* Filter lanes: sep_filter / sep_filter_dev split the random states into

@@ -93,6 +93,7 @@ SIGNATURES = {
     "sep_bench_copy": (_I, [_V, _L, _L, _dp]),
     "sep_set_timing": (_I, [_V, _I]),
     "sep_set_filter_lanes": (_I, [_V, _I]),
+    "sep_trim": (_I, [_V]),
     "sep_autotune": (_I, [_V, _I, _L, _dp, _dp]),
     "sep_filter_batch": (_I, [_I, _V, _V, _lp, _V, _V, _lp, _lp, _dp, _dp, _I, _I]),
     "sep_get_timing": (_I, [_V, _dp, ctypes.POINTER(ctypes.c_long)]),
@@ -384,6 +385,10 @@ class Context:
     def set_timing(self, on):
         """Sample kernel durations inside the filter loop (see sep_set_timing)."""
         self._chk(lib().sep_set_timing(self._h, 1 if on else 0), "sep_set_timing")
+
+    def trim(self):
+        """Free the filter's ring of Newton vectors (sep_trim)."""
+        self._chk(lib().sep_trim(self._h), "sep_trim")
 
     def set_filter_lanes(self, lanes):
         """Concurrent groups of random states in the filter (sep_set_filter_lanes):

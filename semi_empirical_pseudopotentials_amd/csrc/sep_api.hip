@@ -1680,6 +1680,16 @@ extern "C" int sep_set_timing(sep_ctx *c, int on) {
   return SEP_OK;
 }
 
+extern "C" int sep_trim(sep_ctx *c) {
+  if (!c) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->st));  // lanes have joined into st
+  if (c->ring) HIPCHK(c, hipFree(c->ring));
+  c->ring = nullptr;
+  c->ring_bytes = 0;
+  return SEP_OK;
+}
+
 extern "C" int sep_set_filter_lanes(sep_ctx *c, int lanes) {
   if (!c || lanes < 0 || lanes > SEP_MAX_LANES) return SEP_ERR_ARG;
   c->lanes = lanes;

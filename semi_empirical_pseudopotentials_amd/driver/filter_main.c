@@ -250,6 +250,7 @@ int main(int argc, char **argv) {
     t0 = wall();
     c0 = (double)clock();
     CHECK(sep_filter(ctx, psi0, ns, an, zn, nc, ms, Vmin, dE, prec, psif, ene));
+    CHECK(sep_trim(ctx)); /* one filter per run: its ring is not needed again */
     printf("done calculating filter, CPU time (sec) %g, wall run time (sec) %g\n",
            ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
     for (long jns = 0; jns < ns; jns++) { /* filter.c:38-48 */

@@ -74,3 +74,27 @@ def test_copy_probe(sep):
             ctx.bench_copy(1024, 1)
     finally:
         ctx.close()
+
+
+def test_trim_then_filter_again(sep):
+    """sep_trim frees the ring; the next filter allocates it again and gives
+    bitwise the same states."""
+    nc, ms = 300, 3
+    d, psi0 = _setup(sep, 32, 1.7, 4, nc, ms)
+    n = d["nx"]
+    ctx = sep.Context(n, n, n, d["dx"], d["dy"], d["dz"], d["xmin"], d["ymin"], d["zmin"])
+    try:
+        ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
+                            expot=d["expot"])
+        dE, _, _ = ctx.energy_range()
+        zn, _ = sep.leja_points(nc)
+        an = sep.newton_coeffs(zn, np.linspace(d["Elmin"], d["Elmax"], ms), dE, ctx.vmin,
+                               (nc / (2.5 * dE)) ** 2)
+        print("case trim", flush=True)
+        a, _ = ctx.filter(psi0, an, zn, ctx.vmin, dE, sep.SEP_PREC_FP32)
+        ctx.trim()
+        ctx.trim()  # twice: nothing left to free
+        b, _ = ctx.filter(psi0, an, zn, ctx.vmin, dE, sep.SEP_PREC_FP32)
+        assert np.array_equal(a, b)
+    finally:
+        ctx.close()
