@@ -64,3 +64,48 @@ def test_fullsize_fp32_pipeline_within_1mev(sep):
           flush=True)
     assert edge.sum() >= 20
     assert err.max() <= MEV_HA
+
+
+def test_fullsize_filter_vs_oracle(sep, oracle):
+    """At the full 128^3 grid of the 4.0 nm dot (1667 atoms): the device
+    V(r) equals the oracle's to 1e-13 of its range, and a short filter (2
+    random states, 4 targets, 48 Newton steps) of both precisions equals the
+    oracle's: FP64 to 1e-9, complex64 to 2e-4 relative."""
+    import tempfile
+    from semi_empirical_pseudopotentials_amd import decks
+    from semi_empirical_pseudopotentials_amd._lib import read_deck
+    from tests.deckutil import context_for, device_potential, oracle_potential
+    n, ns, ms, nc = 128, 2, 4, 48
+    syms, xyz = decks.cdse_dot(4.0)
+    tmp = tempfile.mkdtemp(prefix="sep_full_o_")
+    c, h = decks.WINDOW_CENTER_HA, decks.WINDOW_HALF_HA
+    decks.write_deck(tmp, syms, xyz, n, n, n, ms, ns, nc, c - h, c + h)
+    d = read_deck(tmp)
+    grid = (n, n, n)
+    print("case fullsize oracle V", flush=True)
+    V = oracle_potential(oracle, d)
+    ksq = oracle.ksqr(n, n, n, d["dx"], d["dy"], d["dz"])
+    r = oracle.Rng()
+    idc = np.array([-1001], dtype=np.int64)
+    psi0 = np.array([r.init_psi(idc, V.size, d["dv"]) for _ in range(ns)])
+    with context_for(sep, d) as ctx:
+        Vd = device_potential(ctx, d)  # built on the device, returned to the host
+        ctx.set_potential(V)
+        dE, _, _ = ctx.energy_range()
+        zn, _ = sep.leja_points(nc)
+        an = sep.newton_coeffs(zn, np.linspace(d["Elmin"], d["Elmax"], ms), dE, V.min(),
+                               (nc / (2.5 * dE)) ** 2)
+        print("case fullsize filters", flush=True)
+        g64, _ = ctx.filter(psi0, an, zn, V.min(), dE, sep.SEP_PREC_FP64)
+        g32, _ = ctx.filter(psi0, an, zn, V.min(), dE, sep.SEP_PREC_FP32)
+    print("fullsize V max |diff|", np.abs(Vd - V).max(), "range", V.max() - V.min(), flush=True)
+    assert np.abs(Vd - V).max() <= 1e-13 * (V.max() - V.min())
+    print("case fullsize oracle filter", flush=True)
+    ref = np.concatenate([oracle.normalize_all(
+        oracle.filter_state(grid, ksq, V, an, zn, 4.0 / dE, V.min(), p), d["dv"]) for p in psi0])
+    scale = np.abs(ref).max()
+    e64 = np.abs(g64 - ref).max() / scale
+    e32 = np.abs(g32 - ref).max() / scale
+    print("fullsize filter rel err fp64", e64, "fp32", e32, flush=True)
+    assert e64 < 1e-9
+    assert e32 < 2e-4
