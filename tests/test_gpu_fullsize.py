@@ -109,3 +109,47 @@ def test_fullsize_filter_vs_oracle(sep, oracle):
     print("fullsize filter rel err fp64", e64, "fp32", e32, flush=True)
     assert e64 < 1e-9
     assert e32 < 2e-4
+
+
+def test_config5_hf_adjoint_vs_fd(sep):
+    """The differentiable configuration's grid (3.0 nm dot, 96^3): the
+    Hellmann-Feynman adjoint's dR of dv sum_g rho V equals central finite
+    differences (h = 1e-6 bohr) of the device V(r) builder to 1e-5 of the
+    largest component (plus the differences' own rounding, which matters for
+    the off-diagonal density of two random states, whose derivatives nearly
+    cancel), for a diagonal and an off-diagonal density."""
+    import tempfile
+    from semi_empirical_pseudopotentials_amd import decks
+    from semi_empirical_pseudopotentials_amd._lib import read_deck
+    from tests.deckutil import context_for
+    n = 96
+    syms, xyz = decks.cdse_dot(3.0)
+    tmp = tempfile.mkdtemp(prefix="sep_c5_")
+    c, h = decks.WINDOW_CENTER_HA, decks.WINDOW_HALF_HA
+    decks.write_deck(tmp, syms, xyz, n, n, n, 8, 8, 64, c - h, c + h)
+    d = read_deck(tmp)
+    rng = np.random.default_rng(11)
+    psi = rng.standard_normal((2, n ** 3))
+    psi /= np.sqrt((psi * psi).sum(1) * d["dv"])[:, None]
+    print("case config5 adjoint", d["natom"], "atoms", flush=True)
+    with context_for(sep, d) as ctx:
+        ctx.states_upload(psi)
+        # expot depends on |r| alone, so it drops out of d/dR
+        dR, _, _ = ctx.hf_adjoint([[0, 0], [0, 1]], d["xyz"], d["type"], d["npot"], d["vr"],
+                                  d["pot"], d["dr"])
+        hstep = 1e-6
+        for p, rho in ((0, psi[0] * psi[0]), (1, psi[0] * psi[1])):
+            big = np.abs(dR[p]).max()
+            for a in (0, d["natom"] // 2, d["natom"] - 1):
+                for k in range(3):
+                    q = []
+                    for sgn in (1, -1):
+                        x = d["xyz"].copy()
+                        x[a, k] += sgn * hstep
+                        V = ctx.build_potential(x, d["type"], d["npot"], d["vr"], d["pot"],
+                                                d["dr"], expot=d["expot"])
+                        q.append(np.dot(rho, V) * d["dv"])
+                    fd = (q[0] - q[1]) / (2 * hstep)
+                    # the differences' own rounding: ~eps sum |rho V| dv / h
+                    noise = 1e-15 * np.dot(np.abs(rho), np.abs(V)) * d["dv"] / hstep
+                    assert abs(fd - dR[p, a, k]) < 1e-5 * big + noise, (p, a, k, fd, dR[p, a, k])
