@@ -11,14 +11,17 @@
  *   or_build_potential    numpy brute-force sum on small grids; reference
  *                         In13_P16 run.dat Vmin/Vmax
  *   or_hamiltonian        numpy restatement; plane-wave eigenvalues
- *   or_ran_nrc            numpy int64 restatement (bitwise)
+ *   or_ran_nrc            numpy int64 restatement (bitwise); with or_init_psi
+ *                         and the filter, the reference's In13_P16
+ *                         eval-filt-*.dat at the recovered seed (1.9e-9 Ha)
  *   or_energy_range       constant-V closed form; In13_P16 run.dat dE, dt
  *   or_leja_points        brute-force greedy on tiny nc; point-set properties
  *   or_newton_coeffs      mpmath divided differences; interpolation identity
  *   or_filter_state       scalar Newton polynomial on plane waves
  *   or_ortho              numpy SVD (span, singular values, cutoff)
  *   or_hmat/or_sigma      scipy eigh; sigma = 0 for exact eigenvectors
- *   or_run                In13_P16 reference eval.dat (converged states)
+ *   or_run                In13_P16 reference eval.dat row by row at the
+ *                         recovered seed (sigma < 1e-2: 5.6e-9 Ha)
  */
 #define _GNU_SOURCE
 #include "sep_oracle.h"

@@ -33,6 +33,9 @@ struct sep_ctx {
   size_t tb_bytes = 0;
   double *scr = nullptr;
   size_t scr_len = 0;
+  // split-K partial sums of the matrix-core Gram (gram_dev)
+  double *gwork = nullptr;
+  size_t gwork_bytes = 0;
   void *ring = nullptr;
   size_t ring_bytes = 0;
   // live kernel timing of the filter loop (sep_set_timing): one sampled
@@ -238,7 +241,7 @@ int sep_close(sep_ctx *c) {
   }
   for (void *p : {(void *)c->gd.kxx, (void *)c->gd.kyy, (void *)c->gd.kzz, (void *)c->pot64,
                   (void *)c->pot32, (void *)c->states, (void *)c->states2, c->tb, (void *)c->scr,
-                  c->ring})
+                  (void *)c->gwork, c->ring})
     if (p) hipFree(p);
   for (hipEvent_t e : c->tev)
     if (e) hipEventDestroy(e);
@@ -1038,62 +1041,41 @@ static int dev_chol_inv(sep_ctx *c, long n, double *G_d, const char *what) {
   return zero_lower_dev(G_d, n, c->st) == SEP_OK ? SEP_OK : fail(c, SEP_ERR_HIP, "%s: zero_lower", what);
 }
 
-// C[i][j] = sum_g A[i][g] B[j][g] (m x mb, row-major, device) for state-major
-// FP64 A (m x N) and B (mb x N): rocBLAS dgemm, no atomics (deterministic),
-// in blocks of at most 512 x 512 -- at K = N ~ 1e6 its kernel choice for a
-// single 1024 x 1024 product ran at a fifth of the 512-block rate (13 vs 60
-// TFLOP/s measured at N = 128^3).  Column-major X (mb x m, ld mb) with
-// X(j, i) = C[i][j] is B^T A.
+// C[i][j] = sum_k A[i*lda + k] B[j*ldb + k] (m x mb, row-major, device):
+// the matrix-core Gram of sep_gemm.hip (f64 MFMA 16x16x4, deterministic
+// split-K).  K = N for states.
+static int sub_gram_k(sep_ctx *c, const double *A, long lda, long m, const double *B, long ldb,
+                      long mb, long K, bool sym, double *C_d) {
+  const long wl = gram_work_len(m, mb, K);
+  RCHK(ensure(c, (void **)&c->gwork, &c->gwork_bytes, sizeof(double) * wl));
+  const int rc = gram_dev<double>(A, lda, m, B, ldb, mb, K, sym, C_d, c->gwork, wl, c->st);
+  return rc == SEP_OK ? SEP_OK : fail(c, rc, "Gram launch failed");
+}
 static int sub_gram(sep_ctx *c, const double *A, long m, const double *B, long mb, double *C_d) {
-  RCHK(dense_init(c, 0));
-  const long N = c->gd.N;
-  const double one = 1.0, zero = 0.0;
-  for (long i0 = 0; i0 < m; i0 += 512)
-    for (long j0 = 0; j0 < mb; j0 += 512) {
-      const long mi = std::min(512L, m - i0), mj = std::min(512L, mb - j0);
-      rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none,
-                                        (rocblas_int)mj, (rocblas_int)mi, (rocblas_int)N, &one,
-                                        B + j0 * N, (rocblas_int)N, A + i0 * N, (rocblas_int)N, &zero,
-                                        C_d + j0 + i0 * mb, (rocblas_int)mb);
-      if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "Gram: rocblas_dgemm status %d", (int)rs);
-    }
-  return SEP_OK;
+  return sub_gram_k(c, A, c->gd.N, m, B, c->gd.N, mb, c->gd.N, false, C_d);
 }
 
 // Symmetric C = A B^T, m x m (B = A for a Gram matrix, B = H A for a
-// projected Hamiltonian, symmetric in exact arithmetic): only the 512-blocks
-// on and above the block diagonal are multiplied, those below take their
-// mirror, and the whole is scaled.
+// projected Hamiltonian, symmetric in exact arithmetic): only the 128-tiles
+// on and above the diagonal are multiplied, those below take their mirror,
+// and the whole is scaled.
 static int sub_gram_sym(sep_ctx *c, const double *A, const double *B, long m, double scale,
                         double *C_d) {
-  RCHK(dense_init(c, 0));
-  const long N = c->gd.N;
-  const double one = 1.0, zero = 0.0;
-  for (long i0 = 0; i0 < m; i0 += 512)
-    for (long j0 = i0; j0 < m; j0 += 512) {
-      const long mi = std::min(512L, m - i0), mj = std::min(512L, m - j0);
-      rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none,
-                                        (rocblas_int)mj, (rocblas_int)mi, (rocblas_int)N, &one,
-                                        B + j0 * N, (rocblas_int)N, A + i0 * N, (rocblas_int)N, &zero,
-                                        C_d + j0 + i0 * m, (rocblas_int)m);
-      if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "Gram: rocblas_dgemm status %d", (int)rs);
-    }
-  return symmetrize_dev(C_d, m, scale, c->st, 512) == SEP_OK ? SEP_OK
+  RCHK(sub_gram_k(c, A, c->gd.N, m, B, c->gd.N, m, c->gd.N, true, C_d));
+  return symmetrize_dev(C_d, m, scale, c->st, 128) == SEP_OK ? SEP_OK
                                                              : fail(c, SEP_ERR_HIP, "symmetrize launch");
 }
 
-// Out[j][g] = sum_i M[i * ldm + j] A[i][g] (mo states out, Out != A):
-// column-major Out (N x mo) = A (N x m) M^T with M column-major (ldm x m).
+// Out[j][g] = sum_i M[i * ldm + j] A[i][g] (mo states out, Out != A): the
+// matrix-core rotation of sep_gemm.hip.
+static int sub_combine_k(sep_ctx *c, const double *A, long lda, long m, const double *M_d,
+                         long ldm, long mo, long N, double *Out, long ldo) {
+  const int rc = combine_dev<double>(A, lda, m, M_d, ldm, mo, N, Out, ldo, c->st);
+  return rc == SEP_OK ? SEP_OK : fail(c, rc, "rotation launch failed");
+}
 static int sub_combine(sep_ctx *c, const double *A, long m, const double *M_d, long ldm, long mo,
                        double *Out) {
-  RCHK(dense_init(c, 0));
-  const long N = c->gd.N;
-  const double one = 1.0, zero = 0.0;
-  rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose,
-                                    (rocblas_int)N, (rocblas_int)mo, (rocblas_int)m, &one, A,
-                                    (rocblas_int)N, M_d, (rocblas_int)ldm, &zero, Out, (rocblas_int)N);
-  return rs == rocblas_status_success ? SEP_OK
-                                      : fail(c, SEP_ERR_HIP, "rotation: rocblas_dgemm status %d", (int)rs);
+  return sub_combine_k(c, A, c->gd.N, m, M_d, ldm, mo, c->gd.N, Out, c->gd.N);
 }
 
 // Cholesky-QR in FP64 on the k resident states: G = A A^T, R = chol(G),
@@ -1266,22 +1248,14 @@ static int rr_polished(sep_ctx *c, long k, double *eval) {
   RCHK(sub_gram_sym(c, c->states, c->states, k, 1.0, U_d));
   RCHK(sub_gram_sym(c, c->states, c->states2, k, 1.0, P_d));
   RCHK(dev_chol_inv(c, k, U_d, "Rayleigh-Ritz: Gram factor"));  // U = R^-1, row-major
-  // Column-major views: U_d holds U^T, P_d holds P (symmetric).
-  // W = P U, then P' = U^T W = R^-T P R^-1 over P_d.
-  const double one = 1.0, zero = 0.0;
-  const rocblas_int n = (rocblas_int)k;
-  rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, n, n,
-                                    n, &one, P_d, n, U_d, n, &zero, W_d, n);
-  if (rs == rocblas_status_success)
-    rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, U_d, n,
-                       W_d, n, &zero, P_d, n);
-  if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "Rayleigh-Ritz: rocblas_dgemm status %d", (int)rs);
+  // row-major k x k products on the matrix cores:
+  // W = P U = P^T U, then P' = U^T W = R^-T P R^-1 over P_d
+  RCHK(sub_combine_k(c, U_d, k, k, P_d, k, k, k, W_d, k));
+  RCHK(sub_combine_k(c, W_d, k, k, U_d, k, k, k, P_d, k));
   RCHK(symmetrize_dev(P_d, k, 1.0, c->st));
-  RCHK(dev_symeig(c, k, P_d, eval, "Rayleigh-Ritz"));  // Q column-major in P_d
-  // coefficients C = U Q, row-major: as column-major it is C^T = Q^T U^T
-  rs = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one, P_d,
-                     n, U_d, n, &zero, W_d, n);
-  if (rs != rocblas_status_success) return fail(c, SEP_ERR_HIP, "Rayleigh-Ritz: rocblas_dgemm status %d", (int)rs);
+  RCHK(dev_symeig(c, k, P_d, eval, "Rayleigh-Ritz"));  // Q column-major in P_d = Q^T row-major
+  // coefficients C = U Q = U (Q^T)^T, a Gram product of the rows of U and Q^T
+  RCHK(sub_gram_k(c, U_d, k, k, P_d, k, k, k, false, W_d));
   RCHK(sub_combine(c, c->states, k, W_d, k, k, c->states2));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
@@ -1687,6 +1661,9 @@ extern "C" int sep_trim(sep_ctx *c) {
   if (c->ring) HIPCHK(c, hipFree(c->ring));
   c->ring = nullptr;
   c->ring_bytes = 0;
+  if (c->gwork) HIPCHK(c, hipFree(c->gwork));
+  c->gwork = nullptr;
+  c->gwork_bytes = 0;
   return SEP_OK;
 }
 
@@ -1711,15 +1688,20 @@ extern "C" int sep_get_timing(sep_ctx *c, double *avg_ms, long *samples) {
 extern "C" int sep_gram_dev(sep_ctx *c, int precision, const void *A, long m, const void *B,
                             long mb, double *C) {
   if (!c || !A || !B || !C || m < 1 || mb < 1) return SEP_ERR_ARG;
+  if (precision != SEP_PREC_FP64 && precision != SEP_PREC_FP32)
+    return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
   HIPCHK(c, hipSetDevice(c->device));
   const long N = c->gd.N;
-  const long gl = gram_work_len(m, mb, N);
-  RCHK(ensure_scr(c, gl + m * mb));
-  double *C_d = c->scr + gl;
-  if (precision == SEP_PREC_FP64)
+  RCHK(ensure_scr(c, m * mb));
+  double *C_d = c->scr;
+  if (precision == SEP_PREC_FP64) {
     RCHK(sub_gram(c, (const double *)A, m, (const double *)B, mb, C_d));
-  else
-    RCHK(gram_dev<float>((const float *)A, m, (const float *)B, mb, N, C_d, c->scr, gl, c->st));
+  } else {
+    const long wl = gram_work_len(m, mb, N);
+    RCHK(ensure(c, (void **)&c->gwork, &c->gwork_bytes, sizeof(double) * wl));
+    RCHK(gram_dev<float>((const float *)A, N, m, (const float *)B, N, mb, N, false, C_d, c->gwork,
+                         wl, c->st));
+  }
   HIPCHK(c, hipMemcpyAsync(C, C_d, sizeof(double) * m * mb, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return SEP_OK;
@@ -1728,13 +1710,16 @@ extern "C" int sep_gram_dev(sep_ctx *c, int precision, const void *A, long m, co
 extern "C" int sep_rotate_dev(sep_ctx *c, int precision, const void *A, long m, const double *M,
                               long mo, void *Out) {
   if (!c || !A || !M || !Out || m < 1 || mo < 1 || A == Out) return SEP_ERR_ARG;
+  if (precision != SEP_PREC_FP64 && precision != SEP_PREC_FP32)
+    return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
   HIPCHK(c, hipSetDevice(c->device));
   RCHK(ensure_scr(c, m * mo));
   HIPCHK(c, hipMemcpyAsync(c->scr, M, sizeof(double) * m * mo, hipMemcpyHostToDevice, c->st));
+  const long N = c->gd.N;
   if (precision == SEP_PREC_FP64)
     RCHK(sub_combine(c, (const double *)A, m, c->scr, mo, mo, (double *)Out));
   else
-    RCHK(combine_dev<float>((const float *)A, m, c->scr, mo, mo, c->gd.N, (float *)Out, c->st));
+    RCHK(combine_dev<float>((const float *)A, N, m, c->scr, mo, mo, N, (float *)Out, N, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return SEP_OK;
 }

@@ -5,8 +5,8 @@
 //   rotation             Out[j][g] = sum_i M[i][j] A[i][g]   (filter/Hmat.c:61-78)
 // plus the double-double Gram that decides the SVD cutoff in FP64 exactly
 // enough to resolve S_i/S_0 = 1e-10 (filter/ortho.c:41-46).  The Gram and
-// rotation kernels serve FP32 operands; the FP64 subspace stage calls
-// rocBLAS dgemm for those two plain products.
+// rotation kernels serve both the FP64 subspace stage (f64 MFMA 16x16x4)
+// and FP32 operands (f32 MFMA 16x16x4).
 //
 // MFMA 16x16x4 operand layout (wave64, lane l):  A[i = l%16][k = l/16],
 // B[k = l/16][j = l%16]; accumulator element r < 4 of lane l holds
@@ -43,63 +43,153 @@ struct Mfma<float> {
 };
 
 // ------------------------------------------------------------- split-K Gram
-// Block: 256 threads, 64x64 tile of C, K chunk `kc` of the grid dimension.
-// Wave w owns the 32x32 quadrant (w/2, w%2) = 2x2 MFMA tiles.
-constexpr int GB = 64, GK = 32;
+// C[i][j] = sum_k A[i*lda + k] B[j*ldb + k] for row-major operands whose rows
+// run along the contraction index (states over grid points, or the rows of
+// a small matrix).  Block: 256 threads = 2 x 2 waves, 128 x 128 tile of C;
+// wave (wi, wj) owns the 64 x 64 quadrant = 4 x 4 MFMA tiles.  The
+// contraction is permuted inside each 16-wide k block: lane group h = l/16
+// takes k0 + 4h + s at MFMA step s (s < 4) for both operands, so a lane
+// reads 4 consecutive elements of each of its 8 rows (one or two 16-byte
+// loads) instead of 4 rows x 1 element -- the product is unchanged, the
+// order of its partial sums is the k permutation.  Split K: chunk z of kc
+// contraction indices writes its partial tile to work[z]; k_sum_chunks adds
+// the chunks in fixed order (deterministic).  Block order is XCD-grouped:
+// hardware dispatch places block b on XCD b % 8, and all tiles of one chunk
+// are given to the same XCD, so their shared A/B rows meet in its L2.
+constexpr int GT = 128;  // block tile
+constexpr int GKB = 16;  // contraction block per MFMA group
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_gram(const R *__restrict__ A, long m,
-                                              const R *__restrict__ B, long mb, long N, long kc,
-                                              double *__restrict__ work) {
-  __shared__ R As[GB][GK + 1];
-  __shared__ R Bs[GB][GK + 1];
-  const long i0 = (long)blockIdx.x * GB, j0 = (long)blockIdx.y * GB;
-  const long g0 = (long)blockIdx.z * kc;
-  long g1 = g0 + kc;
-  if (g1 > N) g1 = N;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wi = w >> 1, wj = w & 1;
-  typename Mfma<R>::acc_t acc[2][2];
+struct Vec4Load {
+  // four consecutive elements at p (16-byte aligned when VEC)
+  template <bool VEC>
+  static __device__ __forceinline__ void load(const R *p, R (&v)[4]) {
+    if constexpr (VEC && sizeof(R) == 4) {
+      const float4 q = *reinterpret_cast<const float4 *>(p);
+      v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+    } else if constexpr (VEC && sizeof(R) == 8) {
+      const double2 q0 = *reinterpret_cast<const double2 *>(p);
+      const double2 q1 = *reinterpret_cast<const double2 *>(p + 2);
+      v[0] = q0.x, v[1] = q0.y, v[2] = q1.x, v[3] = q1.y;
+    } else {
 #pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = typename Mfma<R>::acc_t{0, 0, 0, 0};
-  const int lr = t >> 2, lc = (t & 3) * 8;  // loader: row, 8 consecutive k
-  for (long gk = g0; gk < g1; gk += GK) {
-    __syncthreads();
-    {
-      const long ia = i0 + lr, jb = j0 + lr;
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const long g = gk + lc + q;
-        As[lr][lc + q] = (ia < m && g < g1) ? A[ia * N + g] : (R)0;
-        Bs[lr][lc + q] = (jb < mb && g < g1) ? B[jb * N + g] : (R)0;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < GK; kk += 4) {
-      R a[2], b[2];
-#pragma unroll
-      for (int x = 0; x < 2; x++) {
-        a[x] = As[32 * wi + 16 * x + (lane & 15)][kk + (lane >> 4)];
-        b[x] = Bs[32 * wj + 16 * x + (lane & 15)][kk + (lane >> 4)];
-      }
-#pragma unroll
-      for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++) acc[x][y] = Mfma<R>::op(a[x], b[y], acc[x][y]);
+      for (int s = 0; s < 4; s++) v[s] = p[s];
     }
   }
-  double *wz = work + (long)blockIdx.z * m * mb;
+};
+
+// tile t of the T(T+1)/2 tiles on and above the diagonal of a T x T grid
+__device__ __forceinline__ void upper_tile(int t, int T, int &bi, int &bj) {
+  int i = 0;
+  while (t >= T - i) {
+    t -= T - i;
+    i++;
+  }
+  bi = i;
+  bj = i + t;
+}
+
+template <typename R, bool VEC, int TJ, int ST>
+__global__ void __launch_bounds__(256) k_gram_t(const R *__restrict__ A, long lda, long m,
+                                                const R *__restrict__ B, long ldb, long mb,
+                                                long K, long kc, int tj, int ntiles, int nch,
+                                                int sym, double *__restrict__ work) {
+  // wave tile 64 x 16 TJ, block tile 128 x 32 TJ (2 x 2 waves)
+  constexpr int BW = 32 * TJ;
+  // XCD-grouped decode: block b -> (chunk, tile)
+  const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
+  const int cgrp = slot / ntiles, tile = slot - cgrp * ntiles;
+  const int z = xcd + 8 * cgrp;
+  if (z >= nch) return;
+  int bi, bj;
+  if (sym)
+    upper_tile(tile, tj, bi, bj);
+  else
+    bi = tile / tj, bj = tile - (tile / tj) * tj;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, h = lane >> 4;
+  const long i0 = (long)bi * GT + 64 * (w >> 1), j0 = (long)bj * BW + 16 * TJ * (w & 1);
+  const long g0 = (long)z * kc;
+  const long g1 = g0 + kc < K ? g0 + kc : K;
+  // rows outside the operands read row 0 (their products land in C
+  // entries that are never stored)
+  const R *ap[4], *bp[TJ];
 #pragma unroll
-  for (int x = 0; x < 2; x++)
+  for (int x = 0; x < 4; x++) {
+    const long i = i0 + 16 * x + li;
+    ap[x] = A + (i < m ? i : 0) * lda + 4 * h;
+  }
 #pragma unroll
-    for (int y = 0; y < 2; y++)
+  for (int y = 0; y < TJ; y++) {
+    const long j = j0 + 16 * y + li;
+    bp[y] = B + (j < mb ? j : 0) * ldb + 4 * h;
+  }
+  typename Mfma<R>::acc_t acc[4][TJ];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < TJ; y++) acc[x][y] = typename Mfma<R>::acc_t{0, 0, 0, 0};
+  // ST-stage register pipeline: the loads of block n + ST - 1 are issued
+  // before the MFMAs of block n
+  R a[ST][4][4], bb[ST][TJ][4];
+  auto load = [&](int st, long g) {
+#pragma unroll
+    for (int x = 0; x < 4; x++) Vec4Load<R>::template load<VEC>(ap[x] + g, a[st][x]);
+#pragma unroll
+    for (int y = 0; y < TJ; y++) Vec4Load<R>::template load<VEC>(bp[y] + g, bb[st][y]);
+  };
+  auto step = [&](int st) {
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < TJ; y++)
+          acc[x][y] = Mfma<R>::op(a[st][x][s], bb[st][y][s], acc[x][y]);
+  };
+  long g = g0;
+  const long gfull = g0 + ((g1 - g0) / GKB) * GKB;
+  if (g < gfull) {
+    // loads past the last full block re-read the last one (never used):
+    // unconditional loads keep the wait counts uniform, so the MFMAs of
+    // block n wait only for block n's loads
+    const long glast = gfull - GKB;
+#pragma unroll
+    for (int st = 0; st < ST - 1; st++) load(st, g + st * GKB < gfull ? g + st * GKB : glast);
+    bool done = false;
+    while (!done) {
+#pragma unroll
+      for (int u = 0; u < ST; u++) {
+        if (!done) {
+          const long gl = g + (ST - 1) * GKB;
+          load((u + ST - 1) % ST, gl < gfull ? gl : glast);
+          step(u);
+          g += GKB;
+          done = g >= gfull;
+        }
+      }
+    }
+  }
+  if (g < g1) {  // partial last block, element-wise with zero fill
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const bool in = g + 4 * h + s < g1;
+#pragma unroll
+      for (int x = 0; x < 4; x++) a[0][x][s] = in ? ap[x][g + s] : (R)0;
+#pragma unroll
+      for (int y = 0; y < TJ; y++) bb[0][y][s] = in ? bp[y][g + s] : (R)0;
+    }
+    step(0);
+  }
+  double *wz = work + (long)z * m * mb;
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < TJ; y++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const long i = i0 + 32 * wi + 16 * x + Mfma<R>::row(lane, r);
-        const long j = j0 + 32 * wj + 16 * y + (lane & 15);
+        const long i = i0 + 16 * x + Mfma<R>::row(lane, r);
+        const long j = j0 + 16 * y + li;
         if (i < m && j < mb) wz[i * mb + j] = (double)acc[x][y][r];
       }
 }
@@ -112,109 +202,252 @@ __global__ void k_sum_chunks(const double *work, long nchunk, long len, double *
   }
 }
 
-static long gram_chunk(long N) {
-  long kc = 65536;
-  while (kc > 4096 && (N + kc - 1) / kc < 8) kc /= 2;
-  return kc;
+// Shape of one Gram launch.  Block tile GT x bw; chunk count a multiple of 8
+// (one group per XCD) giving >= ~2048 blocks, chunks of at least 4096
+// contraction indices.
+struct GramShape {
+  int tj, ntiles, nch;
+  long kc;
+};
+static GramShape gram_shape(long m, long mb, long K, bool sym, int bw) {
+  GramShape g;
+  const int ti = (int)((m + GT - 1) / GT);
+  if (sym) bw = GT;  // square tiles on the diagonal
+  g.tj = (int)((mb + bw - 1) / bw);
+  g.ntiles = sym ? ti * (ti + 1) / 2 : ti * g.tj;
+  long want = (2048 + g.ntiles - 1) / g.ntiles;
+  long maxch = (K + 4095) / 4096;
+  if (want > maxch) want = maxch;
+  if (want < 1) want = 1;
+  want = (want + 7) / 8 * 8;
+  g.kc = (K + want - 1) / want;
+  g.kc = (g.kc + GKB - 1) / GKB * GKB;
+  g.nch = (int)((K + g.kc - 1) / g.kc);
+  return g;
 }
 
-long gram_work_len(long m, long mb, long N) {
-  long kc = gram_chunk(N);
-  return ((N + kc - 1) / kc) * m * mb;
-}
-
+// kernel configuration per precision (wave tile width TJ, pipeline depth
+// ST), from a sweep at m = 512, N = 128^3 on MI355X (TFLOP/s, f64 / f32):
+// TJ 4: ST 2 32-43 / 58-63, ST 3 45 / 66-71, ST 4 44 / 70; TJ 2 (two
+// waves per SIMD): 23-32 / 48-52
 template <typename R>
-int gram_dev(const R *A, long m, const R *B, long mb, long N, double *C_d, double *work,
-             long work_len, hipStream_t st) {
-  long kc = gram_chunk(N);
-  long nch = (N + kc - 1) / kc;
-  if (nch * m * mb > work_len) return SEP_ERR_NOMEM;
-  dim3 grid((unsigned)((m + GB - 1) / GB), (unsigned)((mb + GB - 1) / GB), (unsigned)nch);
-  hipLaunchKernelGGL(k_gram<R>, grid, dim3(256), 0, st, A, m, B, mb, N, kc, work);
-  long len = m * mb;
+struct GramCfg;
+template <>
+struct GramCfg<double> {
+  static constexpr int TJ = 4, ST = 3;
+};
+template <>
+struct GramCfg<float> {
+  static constexpr int TJ = 4, ST = 3;
+};
+
+long gram_work_len(long m, long mb, long K) {
+  long n = 0;
+  for (int bw : {32, 64, 128})
+    for (bool sym : {false, true}) {
+      const GramShape g = gram_shape(m, mb, K, sym, bw);
+      if (g.nch > n) n = g.nch;
+    }
+  return n * m * mb;
+}
+
+template <typename R, int TJ, int ST>
+int gram_launch(const R *A, long lda, long m, const R *B, long ldb, long mb, long K, bool sym,
+                double *C_d, double *work, long work_len, hipStream_t st) {
+  if (sym && (m != mb || TJ != 4)) return SEP_ERR_ARG;
+  const GramShape g = gram_shape(m, mb, K, sym, 32 * TJ);
+  if ((long)g.nch * m * mb > work_len) return SEP_ERR_NOMEM;
+  const int ngrp = (g.nch + 7) / 8;
+  const unsigned nblk = (unsigned)(8 * ngrp * g.ntiles);
+  const bool vec = lda % 4 == 0 && ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(A) % 16) == 0 &&
+                   (reinterpret_cast<uintptr_t>(B) % 16) == 0;
+  if (vec)
+    hipLaunchKernelGGL((k_gram_t<R, true, TJ, ST>), dim3(nblk), dim3(256), 0, st, A, lda, m, B,
+                       ldb, mb, K, g.kc, g.tj, g.ntiles, g.nch, sym ? 1 : 0, work);
+  else
+    hipLaunchKernelGGL((k_gram_t<R, false, TJ, ST>), dim3(nblk), dim3(256), 0, st, A, lda, m, B,
+                       ldb, mb, K, g.kc, g.tj, g.ntiles, g.nch, sym ? 1 : 0, work);
+  const long len = m * mb;
   unsigned nb = (unsigned)((len + 255) / 256);
   if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(k_sum_chunks, dim3(nb), dim3(256), 0, st, work, nch, len, C_d);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(nb), dim3(256), 0, st, work, (long)g.nch, len, C_d);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
-template int gram_dev<float>(const float *, long, const float *, long, long, double *, double *,
-                             long, hipStream_t);
-
-// ----------------------------------------------------------------- rotation
-// Out[j][g] = sum_i M[i*ldm + j] A[i][g]; block tile 64 (j) x 128 (g),
-// reduction chunks of 16 states; wave w owns g columns [32w, 32w+32).
-constexpr int CJ = 64, CG = 128, CI = 16;
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_combine(const R *__restrict__ A, long m,
-                                                 const double *__restrict__ M, long ldm, long mo,
-                                                 long N, R *__restrict__ Out) {
-  __shared__ R Ms[CI][CJ + 1];  // Ms[i][j]
-  __shared__ R As[CI][CG + 4];
-  const long j0 = (long)blockIdx.y * CJ, g0 = (long)blockIdx.x * CG;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  typename Mfma<R>::acc_t acc[4][2];
+int gram_dev(const R *A, long lda, long m, const R *B, long ldb, long mb, long K, bool sym,
+             double *C_d, double *work, long work_len, hipStream_t st) {
+  if (sym && m != mb) return SEP_ERR_ARG;
+  return gram_launch<R, GramCfg<R>::TJ, GramCfg<R>::ST>(A, lda, m, B, ldb, mb, K, sym, C_d, work,
+                                                        work_len, st);
+}
+template int gram_dev<float>(const float *, long, long, const float *, long, long, long, bool,
+                             double *, double *, long, hipStream_t);
+template int gram_dev<double>(const double *, long, long, const double *, long, long, long, bool,
+                              double *, double *, long, hipStream_t);
+
+// ----------------------------------------------------------------- rotation
+// Out[j*ldo + g] = sum_i M[i*ldm + j] A[i*lda + g] for i < m, j < mo, g < N:
+// D = M^T A on the matrix cores, M^T the first MFMA operand (16 j x 4 i), A
+// the second (4 i x 16 g).  Wave tile 64 (j) x 64 (g) = 4 x 4 MFMA tiles;
+// block = 4 waves on one 64-point g tile and 4 consecutive j tiles, so A's
+// rows are shared through the CU's cache.  The contraction over i is
+// permuted as in the Gram (lane group h takes i0 + 4h + s at step s), and
+// each lane reads two consecutive points per A row: column c of g-tile
+// pair (2p, 2p + 1) stands for points g0 + 32p + 2c and + 1, so a lane's
+// two results land in one 16-byte store.
+template <typename R, bool VEC, int TG, int ST>
+__global__ void __launch_bounds__(256) k_combine_t(const R *__restrict__ A, long lda, long m,
+                                                   const double *__restrict__ M, long ldm,
+                                                   long mo, long N, R *__restrict__ Out,
+                                                   long ldo) {
+  // wave tile 64 (j) x 16 TG (g); TG even
+  constexpr int NP = TG / 2;  // point pairs per lane
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, h = lane >> 4;
+  const long g0 = (long)blockIdx.x * 16 * TG;
+  const long j0 = ((long)blockIdx.y * 4 + w) * 64;
+  if (j0 >= mo) return;
+  typename Mfma<R>::acc_t acc[4][TG];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = typename Mfma<R>::acc_t{0, 0, 0, 0};
-  for (long i0 = 0; i0 < m; i0 += CI) {
-    __syncthreads();
-    // A tile: 16 x 128, 8 per thread along g
-    {
-      const int r = t >> 4, c = (t & 15) * 8;
-      const long i = i0 + r;
+    for (int y = 0; y < TG; y++) acc[x][y] = typename Mfma<R>::acc_t{0, 0, 0, 0};
+  // this lane's points: g0 + 32p + 2li + e (p < NP, e < 2)
+  long gp[NP];
+  bool gin[NP][2];
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const long g = g0 + c + q;
-        As[r][c + q] = (i < m && g < N) ? A[i * N + g] : (R)0;
+  for (int p = 0; p < NP; p++) {
+    gp[p] = g0 + 32 * p + 2 * li;
+    gin[p][0] = gp[p] < N;
+    gin[p][1] = gp[p] + 1 < N;
+  }
+  const bool full_g = g0 + 16 * TG <= N;
+  long jc[4];
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
+    const long j = j0 + 16 * x + li;
+    jc[x] = j < mo ? j : -1;
+  }
+  R mv[ST][4][4], av[ST][TG][4];  // mv[x][s]: M^T(j, i), av[y][s]: A(i, g)
+  auto load = [&](int st, long i0) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const long i = i0 + 4 * h + s;
+      const bool iin = i < m;
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+        mv[st][x][s] = (iin && jc[x] >= 0) ? (R)M[i * ldm + jc[x]] : (R)0;
+      const R *ar = A + (iin ? i : 0) * lda;
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        R e0, e1;
+        if (VEC && full_g) {
+          if constexpr (sizeof(R) == 8) {
+            const double2 q = *reinterpret_cast<const double2 *>(ar + gp[p]);
+            e0 = q.x, e1 = q.y;
+          } else {
+            const float2 q = *reinterpret_cast<const float2 *>(ar + gp[p]);
+            e0 = q.x, e1 = q.y;
+          }
+        } else {
+          e0 = gin[p][0] ? ar[gp[p]] : (R)0;
+          e1 = gin[p][1] ? ar[gp[p] + 1] : (R)0;
+        }
+        av[st][2 * p][s] = iin ? e0 : (R)0;
+        av[st][2 * p + 1][s] = iin ? e1 : (R)0;
       }
     }
-    // M tile: 16 x 64, 4 per thread
-    {
-      const int r = t >> 4, c = (t & 15) * 4;
-      const long i = i0 + r;
+  };
+  auto step = [&](int st) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const long j = j0 + c + q;
-        Ms[r][c + q] = (i < m && j < mo) ? (R)M[i * ldm + j] : (R)0;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < CI; kk += 4) {
-      R a[4], b[2];
-#pragma unroll
-      for (int x = 0; x < 4; x++) a[x] = Ms[kk + (lane >> 4)][16 * x + (lane & 15)];
-#pragma unroll
-      for (int y = 0; y < 2; y++) b[y] = As[kk + (lane >> 4)][32 * w + 16 * y + (lane & 15)];
+    for (int s = 0; s < 4; s++)
 #pragma unroll
       for (int x = 0; x < 4; x++)
 #pragma unroll
-        for (int y = 0; y < 2; y++) acc[x][y] = Mfma<R>::op(a[x], b[y], acc[x][y]);
+        for (int y = 0; y < TG; y++)
+          acc[x][y] = Mfma<R>::op(mv[st][x][s], av[st][y][s], acc[x][y]);
+  };
+  long i0 = 0;
+  // blocks past the end re-read block 0 (never used): unconditional loads
+  // keep the wait counts uniform
+#pragma unroll
+  for (int st = 0; st < ST - 1; st++) load(st, st * GKB < m ? st * GKB : 0);
+  bool done = false;
+  while (!done) {
+#pragma unroll
+    for (int u = 0; u < ST; u++) {
+      if (!done) {
+        const long il = i0 + (ST - 1) * GKB;
+        load((u + ST - 1) % ST, il < m ? il : 0);
+        step(u);
+        i0 += GKB;
+        done = i0 >= m;
+      }
     }
   }
 #pragma unroll
   for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int y = 0; y < 2; y++)
+    for (int r = 0; r < 4; r++) {
+      const long j = j0 + 16 * x + Mfma<R>::row(lane, r);
+      if (j >= mo) continue;
+      R *op = Out + j * ldo;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const long j = j0 + 16 * x + Mfma<R>::row(lane, r);
-        const long g = g0 + 32 * w + 16 * y + (lane & 15);
-        if (j < mo && g < N) Out[j * N + g] = acc[x][y][r];
+      for (int p = 0; p < NP; p++) {
+        const R e0 = (R)acc[x][2 * p][r], e1 = (R)acc[x][2 * p + 1][r];
+        if (VEC && full_g) {
+          if constexpr (sizeof(R) == 8)
+            *reinterpret_cast<double2 *>(op + gp[p]) = double2{e0, e1};
+          else
+            *reinterpret_cast<float2 *>(op + gp[p]) = float2{e0, e1};
+        } else {
+          if (gin[p][0]) op[gp[p]] = e0;
+          if (gin[p][1]) op[gp[p] + 1] = e1;
+        }
       }
+    }
+}
+
+// from the same sweep (TFLOP/s, f64 / f32): TG 4: ST 1 37-50 / 47-55, ST 2
+// 38-41 / 47; TG 2: ST 1-3 40-45 / 37
+template <typename R>
+struct CombineCfg;
+template <>
+struct CombineCfg<double> {
+  static constexpr int TG = 4, ST = 1;
+};
+template <>
+struct CombineCfg<float> {
+  static constexpr int TG = 4, ST = 1;
+};
+
+template <typename R, int TG, int ST>
+int combine_launch(const R *A, long lda, long m, const double *M_d, long ldm, long mo, long N,
+                   R *Out, long ldo, hipStream_t st) {
+  dim3 grid((unsigned)((N + 16 * TG - 1) / (16 * TG)), (unsigned)((mo + 255) / 256));
+  const bool vec = lda % 2 == 0 && ldo % 2 == 0 &&
+                   (reinterpret_cast<uintptr_t>(A) % (2 * sizeof(R))) == 0 &&
+                   (reinterpret_cast<uintptr_t>(Out) % (2 * sizeof(R))) == 0;
+  if (vec)
+    hipLaunchKernelGGL((k_combine_t<R, true, TG, ST>), grid, dim3(256), 0, st, A, lda, m, M_d, ldm,
+                       mo, N, Out, ldo);
+  else
+    hipLaunchKernelGGL((k_combine_t<R, false, TG, ST>), grid, dim3(256), 0, st, A, lda, m, M_d,
+                       ldm, mo, N, Out, ldo);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 
 template <typename R>
-int combine_dev(const R *A, long m, const double *M_d, long ldm, long mo, long N, R *Out,
-                hipStream_t st) {
-  dim3 grid((unsigned)((N + CG - 1) / CG), (unsigned)((mo + CJ - 1) / CJ));
-  hipLaunchKernelGGL(k_combine<R>, grid, dim3(256), 0, st, A, m, M_d, ldm, mo, N, Out);
-  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+int combine_dev(const R *A, long lda, long m, const double *M_d, long ldm, long mo, long N, R *Out,
+                long ldo, hipStream_t st) {
+  return combine_launch<R, CombineCfg<R>::TG, CombineCfg<R>::ST>(A, lda, m, M_d, ldm, mo, N, Out,
+                                                                 ldo, st);
 }
-template int combine_dev<float>(const float *, long, const double *, long, long, long, float *,
-                                hipStream_t);
+template int combine_dev<float>(const float *, long, long, const double *, long, long, long,
+                                float *, long, hipStream_t);
+template int combine_dev<double>(const double *, long, long, const double *, long, long, long,
+                                 double *, long, hipStream_t);
 
 // ------------------------------------------------ Newton accumulation GEMM
 // acc[s][ie][g] += sum_{k < cnt} anT[j0 + k][ie] * ring[k][s][g]
@@ -538,11 +771,21 @@ __global__ void k_sum_chunks_dd(const double *whi, const double *wlo, long nchun
   }
 }
 
-long gram_dd_work_len(long m, long N) { return 2 * gram_work_len(m, m, N); }
+// contraction chunk of the double-double Gram: >= 8 chunks where N allows
+static long dd_chunk(long N) {
+  long kc = 65536;
+  while (kc > 4096 && (N + kc - 1) / kc < 8) kc /= 2;
+  return kc;
+}
+
+long gram_dd_work_len(long m, long N) {
+  const long kc = dd_chunk(N);
+  return 2 * ((N + kc - 1) / kc) * m * m;
+}
 
 int gram_dd_dev(const double *A, long m, long N, double *hi_d, double *lo_d, double *work,
                 long work_len, hipStream_t st) {
-  long kc = gram_chunk(N);
+  long kc = dd_chunk(N);
   long nch = (N + kc - 1) / kc;
   if (2 * nch * m * m > work_len) return SEP_ERR_NOMEM;
   double *whi = work, *wlo = work + nch * m * m;

@@ -100,16 +100,22 @@ void accum_pack_f32(const double *an, long nc, int ms, float *cf);
 int accum_mfma_packed_dev(const float *ring, long slot, long sstride, int cnt, const float *cf_d,
                           long nc, long j0, int ms, double *acc, int ns, long N, hipStream_t st);
 
-// ---- GEMMs (sep_gemm.hip); all operands state-major (row i = one state)
-// C[i][j] = sum_g A[i][g] B[j][g]   (m x mb), deterministic split-K
+// ---- GEMMs on the matrix cores (sep_gemm.hip); row-major operands whose
+// rows run along the contraction index (state-major states, or the rows of
+// a small matrix).
+// C[i][j] = sum_k A[i*lda + k] B[j*ldb + k]  (m x mb into C_d, row-major),
+// deterministic split-K through work (gram_work_len doubles); sym: m == mb
+// and only the 128-tiles on and above the diagonal are formed (the caller
+// mirrors them, symmetrize_dev with blk = 128)
 template <typename R>
-int gram_dev(const R *A, long m, const R *B, long mb, long N, double *C_d, double *work,
-             long work_len, hipStream_t st);
-// Out[j][g] = sum_i M[i*ldm + j] A[i][g]  (mo states out), Out != A
+int gram_dev(const R *A, long lda, long m, const R *B, long ldb, long mb, long K, bool sym,
+             double *C_d, double *work, long work_len, hipStream_t st);
+long gram_work_len(long m, long mb, long K);
+// Out[j*ldo + g] = sum_i M[i*ldm + j] A[i*lda + g]  (mo rows out, N columns),
+// Out must not overlap A
 template <typename R>
-int combine_dev(const R *A, long m, const double *M_d, long ldm, long mo, long N, R *Out,
-                hipStream_t st);
-long gram_work_len(long m, long mb, long N);
+int combine_dev(const R *A, long lda, long m, const double *M_d, long ldm, long mo, long N, R *Out,
+                long ldo, hipStream_t st);
 // G = A A^T as double-double (hi + lo), m x m row-major: the upper triangle
 // (16 x 16 tiles on and above the diagonal); entries below it are undefined
 int gram_dd_dev(const double *A, long m, long N, double *hi_d, double *lo_d, double *work,

@@ -73,17 +73,90 @@ def test_in13_potential_extrema(oracle, sep):
     assert abs(V.min() - (-1.44656)) < 5e-6 and abs(V.max() - 1.35926) < 5e-6
 
 
+# The reference's In13_P16 directory holds the outputs of one filter run:
+# eval.dat and the 16 per-random-state files eval-filt-<tid>-<jns>.dat
+# (filter/filter.c:38-42: filtered-state Rayleigh quotients).  Its run.dat logs
+# two other runs (seeds -1131359211 and -500037525, run.dat:53,117); the
+# outputs belong to a third, unlogged one whose seed was recovered by running
+# the oracle over the candidate seeds: the reference draws the seed as
+# -random() after skipping time(0) % 1000 values of glibc random()
+# (filter/rand.c:70-78, main.c:79), so every seed is one of the first 1000
+# outputs of random() from its default state.
+IN13_SEED = -822262754
+IN13_LOGGED_SEEDS = (-1131359211, -500037525)
+
+
+def _glibc_random(n):
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.random.restype = ctypes.c_long
+    libc.srandom(1)
+    return [libc.random() for _ in range(n)]
+
+
+def test_in13_seed_provenance():
+    """All three seeds are among the first 1000 outputs of glibc random() from
+    srandom(1), as Randomize (filter/rand.c:70-78) produces them: positions
+    756 and 919 for the logged runs, 389 for the run that wrote the outputs."""
+    r = _glibc_random(1000)
+    assert r.index(-IN13_LOGGED_SEEDS[0]) == 756
+    assert r.index(-IN13_LOGGED_SEEDS[1]) == 919
+    assert r.index(-IN13_SEED) == 389
+
+
+def read_filt_fixtures(path):
+    """{jns: (energies, targets)} of the reference eval-filt-<tid>-<jns>.dat."""
+    out = {}
+    for f in os.listdir(path):
+        if f.startswith("eval-filt-") and f.endswith(".dat"):
+            jns = int(f[:-4].split("-")[-1])
+            a = np.loadtxt(os.path.join(path, f), ndmin=2)
+            out[jns] = (a[:, 1], a[:, 2])
+    return out
+
+
 @pytest.mark.slow
 def test_in13_oracle_end_to_end(oracle, tmp_path):
-    """The oracle reproduces the reference run: dE, dt (run.dat:50-52), the
-    SVD cutoff (run.dat:57) and every converged level of eval.dat to 1e-8 Ha
-    (the strain scales in strain.dat carry 8 decimals)."""
-    s = oracle.run(IN13, str(tmp_path), -1131359211, 0)
+    """The oracle reproduces the reference binary's own In13_P16 run at the
+    recovered seed: dE, dt (run.dat:50-52), the SVD cutoff (run.dat:57), all
+    160 filtered-state energies of eval-filt-*.dat to 5e-9 Ha (measured
+    1.9e-9), and eval.dat row by row: every level with sigma < 1e-2 to 1e-8 Ha
+    (measured 5.6e-9), the rest to 1e-4 Ha (measured 2.7e-5; those Ritz pairs
+    live in directions whose singular values sit near the 1e-10 cutoff and
+    are not determined beyond that).  The residual ~1e-9 comes from the 8
+    decimals of the per-atom scales in strain.dat."""
+    s = oracle.run(IN13, str(tmp_path), IN13_SEED, 0)
     assert abs(s["dE"] - 13.3627) < 5e-5 and abs(s["dt"] - 939.57) < 5e-3
     assert s["cutoff"] == 158
+    ref = read_filt_fixtures(IN13)
+    assert sorted(ref) == list(range(16))
+    worst = 0.0
+    for jns, (e_ref, el_ref) in ref.items():
+        a = np.loadtxt(str(tmp_path / f"eval-filt-0-{jns}.dat"), ndmin=2)
+        assert np.abs(a[:, 2] - el_ref).max() < 1e-15
+        worst = max(worst, np.abs(a[:, 1] - e_ref).max())
+    print("eval-filt max |dE|", worst)
+    assert worst < 5e-9
     ref_e, ref_s = read_eval(os.path.join(IN13, "eval.dat"))
     got_e, got_s = read_eval(str(tmp_path / "eval.dat"))
     assert got_e.size == ref_e.size == 158
+    conv = ref_s < 1e-2
+    assert conv.sum() >= 80
+    d = np.abs(got_e - ref_e)
+    print("eval.dat converged max", d[conv].max(), "rest max", d[~conv].max())
+    assert d[conv].max() < 1e-8
+    assert d[~conv].max() < 1e-4
+
+
+@pytest.mark.slow
+def test_in13_oracle_logged_seed(oracle, tmp_path):
+    """At a logged seed the random states differ from the run that wrote the
+    outputs, but the converged levels (a property of H alone) still match
+    eval.dat to 1e-8 Ha and the cutoff is 158 (run.dat:57)."""
+    s = oracle.run(IN13, str(tmp_path), IN13_LOGGED_SEEDS[0], 0)
+    assert s["cutoff"] == 158
+    ref_e, ref_s = read_eval(os.path.join(IN13, "eval.dat"))
+    got_e, got_s = read_eval(str(tmp_path / "eval.dat"))
     pairs = match_levels(ref_e, ref_s, got_e, 1e-6)
     assert len(pairs) >= 40
     assert np.abs(pairs[:, 0] - pairs[:, 1]).max() < 1e-8
