@@ -82,6 +82,21 @@ int sep_hnorm(sep_ctx *ctx, const double *in_c128, double *out_c128, long batch,
  * precision SEP_PREC_FP64 = complex128, SEP_PREC_FP32 = complex64 data. */
 int sep_apply_h_dev(sep_ctx *ctx, const void *in_dev, void *out_dev, long batch, int precision);
 
+/* No counterpart in the reference: sep_apply_h_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and result as
+ * sep_apply_h_dev, but the FFT spectrum buffer (batch * N complex elements of
+ * the precision) comes from a static device array of SEP_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds from library load to process end; no hipMalloc,
+ * hipFree or allocating library call is made.  batch * N * (16 | 8) bytes
+ * beyond that limit returns SEP_ERR_ARG.  Every call on one GPU shares that
+ * GPU's array: calls that overlap in time on one GPU (other host threads,
+ * other contexts' streams) race.  The potential must be set. */
+#define SEP_NOALLOC_SCRATCH_BYTES (32L << 20)
+#define SEP_MAX_DEVICES 64
+int sep_apply_h_dev_noalloc(sep_ctx *ctx, const void *in_dev, void *out_dev, long batch,
+                            int precision);
+
 /* ---------------------------------------------------------- spectral range */
 /* Power iteration from the fixed-seed state (energy.c:57,61): returns
  * dE = 1.1 (E - vmin) (energy.c:76), the converged E and the iteration count.

@@ -70,6 +70,7 @@ SIGNATURES = {
     "sep_apply_h": (_I, [_V, _dp, _dp, _L]),
     "sep_hnorm": (_I, [_V, _dp, _dp, _L, _D, _D, _D]),
     "sep_apply_h_dev": (_I, [_V, _V, _V, _L, _I]),
+    "sep_apply_h_dev_noalloc": (_I, [_V, _V, _V, _L, _I]),
     "sep_energy_range": (_I, [_V, _D, _dp, _dp, _lp]),
     "sep_rng_reset": (None, [ctypes.POINTER(Rng)]),
     "sep_ran_nrc": (_D, [ctypes.POINTER(Rng), _lp]),

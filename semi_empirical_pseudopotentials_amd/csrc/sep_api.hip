@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <rocsolver/rocsolver.h>
 
+#include <mutex>
+
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -144,6 +146,27 @@ extern "C" {
 
 const char *sep_last_error(const sep_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+// ------------------------------------------- allocation-free H application
+// sep_apply_h_dev_noalloc takes its FFT spectrum buffer from this file-scope
+// device array instead of the context's hipMalloc'd one.  Each GPU holds its
+// own copy from module load to process end; host code reaches it only
+// through the address hipGetSymbolAddress returns with that device current
+// (g_noalloc_ptr), never through the array's name.
+static __device__ __align__(256) unsigned char scratch[SEP_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[SEP_MAX_DEVICES] = {};
+
+// Once per device, with that device current (called from sep_open).
+static int noalloc_init(int device) {
+  if (device < 0 || device >= SEP_MAX_DEVICES) return SEP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return SEP_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SEP_ERR_HIP;
+  g_noalloc_ptr[device] = p;
+  return SEP_OK;
+}
+
 int sep_fft_length_supported(long n) { return hop_supported((int)n) ? 1 : 0; }
 
 int sep_open(const sep_grid_desc *g, sep_ctx **out) {
@@ -162,6 +185,10 @@ int sep_open(const sep_grid_desc *g, sep_ctx **out) {
       hipStreamCreate(&c->lst[0]) != hipSuccess) {
     if (c->st) hipStreamDestroy(c->st);
     delete c;
+    return SEP_ERR_HIP;
+  }
+  if (noalloc_init(c->device) != SEP_OK) {
+    sep_close(c);
     return SEP_ERR_HIP;
   }
   c->gd.nx = (int)g->nx;
@@ -474,6 +501,43 @@ int sep_apply_h_dev(sep_ctx *c, const void *in_dev, void *out_dev, long batch, i
       RCHK(apply_views<float>(c, vi, vo, (int)nb, 1.0, 0.0, nullptr));
     }
   }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return SEP_OK;
+}
+
+int sep_apply_h_dev_noalloc(sep_ctx *c, const void *in_dev, void *out_dev, long batch,
+                            int precision) {
+  if (!c || !in_dev || !out_dev || batch < 1) return SEP_ERR_ARG;
+  if (precision != SEP_PREC_FP64 && precision != SEP_PREC_FP32)
+    return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  const long N = c->gd.N;
+  const size_t per = (precision == SEP_PREC_FP64 ? sizeof(cx<double>) : sizeof(cx<float>)) * (size_t)N;
+  if ((size_t)batch * per > (size_t)SEP_NOALLOC_SCRATCH_BYTES)
+    return fail(c, SEP_ERR_ARG,
+                "apply_h_dev_noalloc: %ld vectors need %zu bytes of scratch, the library holds %ld",
+                batch, (size_t)batch * per, (long)SEP_NOALLOC_SCRATCH_BYTES);
+  void *tb = g_noalloc_ptr[c->device];
+  if (!tb) return fail(c, SEP_ERR_HIP, "apply_h_dev_noalloc: scratch not initialised");
+  HIPCHK(c, hipSetDevice(c->device));
+  const long NB = 64;  // complex vectors per launch, as sep_apply_h_dev
+  int rc = SEP_OK;
+  for (long b0 = 0; rc == SEP_OK && b0 < batch; b0 += NB) {
+    const int nb = (int)(batch - b0 < NB ? batch - b0 : NB);
+    long nblk = 0;
+    if (precision == SEP_PREC_FP64) {
+      VecView<double> vi{nullptr, nullptr, (cx<double> *)in_dev + b0 * N, N};
+      VecView<double> vo{nullptr, nullptr, (cx<double> *)out_dev + b0 * N, N};
+      rc = hpsi_apply<double>(c->gd, vi, vo, reinterpret_cast<cx<double> *>(tb), c->pot64, nb, 1.0,
+                              0.0, nullptr, &nblk, c->st);
+    } else {
+      VecView<float> vi{nullptr, nullptr, (cx<float> *)in_dev + b0 * N, N};
+      VecView<float> vo{nullptr, nullptr, (cx<float> *)out_dev + b0 * N, N};
+      rc = hpsi_apply<float>(c->gd, vi, vo, reinterpret_cast<cx<float> *>(tb), c->pot32, nb, 1.0,
+                             0.0, nullptr, &nblk, c->st);
+    }
+  }
+  if (rc != SEP_OK) return fail(c, rc, "apply_h_dev_noalloc: H.psi launch failed");
   HIPCHK(c, hipStreamSynchronize(c->st));
   return SEP_OK;
 }
