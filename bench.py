@@ -44,14 +44,18 @@ def parse():
                    help="filter lanes (sep_set_filter_lanes): 0 = automatic, 1 = one stream")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-steps", type=int, default=192,
-                   help="Newton steps of the CPU baseline sample (about 10 s on 16 threads)")
+    p.add_argument("--cpu-steps", type=int, default=24,
+                   help="Newton steps of the CPU baseline sample, all random states in "
+                        "parallel (about 20 s on 16 threads)")
     p.add_argument("--pipeline", action=argparse.BooleanOptionalAction, default=True,
                    help="time the whole filter-diagonalization pipeline of both precisions")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched uneven-grid measurement (8 dots, 64-160^3)")
     p.add_argument("--no-check", action="store_true",
                    help="skip the FP32-pipeline accuracy check (2 nm dot, 64^3)")
+    p.add_argument("--gemm-only", action="store_true",
+                   help="only time the subspace-stage matrix-core products (Gram and rotation, "
+                        "FP64 and FP32, m = 512 states at the grid); for rocprofv3 --pmc runs")
     p.add_argument("--kernels-only", action="store_true",
                    help="only launch the H.psi passes (FP32, 5 times) and a 256 MiB device "
                         "copy that calibrates the HBM byte counters; for rocprofv3 --pmc runs")
@@ -75,6 +79,45 @@ def measured_traffic(kernel, n, nvec):
         if line.startswith("traffic_bytes_per_launch") and kernel in line:
             return int(line.split()[-1])
     return None
+
+
+def subspace_gemms(sep, np, torch, n, m=512, reps=3):
+    """The subspace stage's N x m products on the library's MFMA kernels
+    (filter/Hmat.c:30-46 projected H / ortho.c:32 Gram, Hmat.c:61-78
+    rotation) at m states of N = n^3 points: the Gram C = A B^T
+    (sep_gram_dev) and the rotation Out = M^T A (sep_rotate_dev), FP64 and
+    FP32 operands, 2 m^2 N flop each.  Times include the m x m transfer."""
+    out = {"m": m, "N": n ** 3, "flop_per_product": 2.0 * m * m * n ** 3}
+    with sep.Context(n, n, n, 0.5, 0.5, 0.5) as ctx:
+        N = ctx.N
+        g = torch.Generator(device="cuda:0").manual_seed(7)
+        for name, dt, P in (("fp64", torch.float64, sep.SEP_PREC_FP64),
+                            ("fp32", torch.float32, sep.SEP_PREC_FP32)):
+            A = torch.randn(m, N, dtype=dt, device="cuda:0", generator=g)
+            B = torch.randn(m, N, dtype=dt, device="cuda:0", generator=g)
+            O = torch.empty(m, N, dtype=dt, device="cuda:0")
+            M = np.random.default_rng(1).standard_normal((m, m))
+            ctx.gram_dev(A.data_ptr(), m, B.data_ptr(), m, P)
+            ctx.rotate_dev(A.data_ptr(), m, M, O.data_ptr(), P)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ctx.gram_dev(A.data_ptr(), m, B.data_ptr(), m, P)
+            tg = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ctx.rotate_dev(A.data_ptr(), m, M, O.data_ptr(), P)
+            tr = (time.perf_counter() - t0) / reps
+            fl = out["flop_per_product"]
+            peak = 78.6 if name == "fp64" else 157.3
+            out[name] = {"gram_ms": round(tg * 1e3, 3), "gram_TFLOPs": round(fl / tg / 1e12, 1),
+                         "rotate_ms": round(tr * 1e3, 3),
+                         "rotate_TFLOPs": round(fl / tr / 1e12, 1),
+                         "peak_TFLOPs": peak,
+                         "gram_frac": round(fl / tg / 1e12 / peak, 3),
+                         "rotate_frac": round(fl / tr / 1e12 / peak, 3)}
+            del A, B, O
+    return out
 
 
 def precision_check(sep, decks, read_deck, np):
@@ -215,6 +258,9 @@ def main():
     from semi_empirical_pseudopotentials_amd._lib import read_deck
 
     n, ns, ms, nc = args.grid, args.ns, args.ms, args.nc
+    if args.gemm_only:
+        print(json.dumps({"gemm_only": True, **subspace_gemms(sep, np, torch, n)}), flush=True)
+        return
     # ---------------------------------------------------------------- setup
     t_setup = time.time()
     syms, xyz = decks.cdse_dot(args.diameter)
@@ -431,16 +477,25 @@ def main():
         V[:] = ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
                                    expot=d["expot"])
         steps = args.cpu_steps
-        p0 = psi0_dev[0].cpu().numpy()
+        p0 = psi0_dev.cpu().numpy()
         t0 = time.perf_counter()
-        O.filter_state(grid, ksq, V, an[:, :steps + 1], zn[:steps + 1], 4.0 / dE, vmin, p0)
+        O.filter_states(grid, ksq, V, an[:, :steps + 1], zn[:steps + 1], 4.0 / dE, vmin, p0)
         tc = time.perf_counter() - t0
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": round(steps / tc, 3), "unit": "H·ψ/s", "cores": threads,
-               "kind": "port",
-               "sample": f"oracle filter (own C FFT, OpenMP), 1 state x {steps} Newton steps "
-                         f"x {ms} targets at {n}^3, {tc:.1f} s"}
-        log(f"cpu oracle: {steps / tc:.2f} H.psi/s on {threads} threads")
+        model = "unknown"
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu = {"value": round(ns * steps / tc, 3), "unit": "H·ψ/s", "cores": threads,
+               "kind": "port", "cpu_model": model, "nproc": os.cpu_count(),
+               "sample": f"oracle filter (own C FFT), all {ns} random states in parallel "
+                         f"(one per OpenMP thread) x {steps} Newton steps x {ms} targets at "
+                         f"{n}^3: {ns * steps} H.psi in {tc:.1f} s"}
+        log(f"cpu oracle: {ns * steps / tc:.2f} H.psi/s on {threads} threads ({model})")
 
     batched = None
     if not args.no_batched:
@@ -449,6 +504,11 @@ def main():
         del psi0_dev
         torch.cuda.empty_cache()
         batched = batched_uneven(sep, decks, read_deck, np, torch)
+    if ctx is not None:
+        ctx.close()
+        ctx = None
+    gemm = subspace_gemms(sep, np, torch, n)
+    log(f"subspace gemms: {gemm}")
 
     res = {
         "metric": METRIC,
@@ -485,6 +545,7 @@ def main():
         "pipeline_128": pipe or None,
         "batched_uneven": batched,
         "hf_adjoint": dict(adj, atoms=d["natom"], states=m),
+        "subspace_gemm": gemm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

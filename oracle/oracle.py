@@ -53,6 +53,8 @@ def lib():
         L.or_newton_coeffs.argtypes = [_L, _L, _dp, _dp, _D, _D, _D, _dp]
         L.or_filter_state.argtypes = [_L, _L, _L, _dp, _dp, _dp, _dp, _L, _L, _D,
                                       _D, _dp, _dp]
+        L.or_filter_states.argtypes = [_L, _L, _L, _dp, _dp, _dp, _dp, _L, _L, _D, _D, _L, _dp,
+                                       _dp]
         L.or_normalize_all.argtypes = [_dp, _D, _L, _L]
         L.or_energy_all.argtypes = [_L, _L, _L, _dp, _dp, _D, _dp, _L, _dp]
         L.or_ortho.argtypes = [_dp, _L, _L, _dp]
@@ -150,6 +152,19 @@ def filter_state(grid, ksq, potl, an, zn, dE_1, vmin, psi0):
     out = np.zeros((ms, nx * ny * nz))
     lib().or_filter_state(nx, ny, nz, _p(_c(ksq)), _p(_c(potl)), _p(an), _p(_c(zn)), nc,
                           ms, dE_1, vmin, _p(_c(psi0)), _p(out))
+    return out
+
+
+def filter_states(grid, ksq, potl, an, zn, dE_1, vmin, psi0):
+    """or_filter_state for every row of psi0 (ns x N), states in parallel;
+    returns (ns, ms, N)."""
+    nx, ny, nz = grid
+    an = _c(an)
+    ms, nc = an.shape
+    p0 = _c(psi0).reshape(-1, nx * ny * nz)
+    out = np.zeros((p0.shape[0], ms, nx * ny * nz))
+    lib().or_filter_states(nx, ny, nz, _p(_c(ksq)), _p(_c(potl)), _p(an), _p(_c(zn)), nc, ms,
+                           dE_1, vmin, p0.shape[0], _p(p0), _p(out))
     return out
 
 

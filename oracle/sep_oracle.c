@@ -695,6 +695,26 @@ int or_filter_state(long nx, long ny, long nz, const double *ksqr,
   return 0;
 }
 
+/* or_filter_state for ns random states at once, one state per thread
+ * (filter/main.c:92-98 runs the states in parallel the same way); out is
+ * [ns][ms][N]. */
+int or_filter_states(long nx, long ny, long nz, const double *ksqr, const double *potl,
+                     const double *an, const double *zn, long nc, long ms, double dE_1,
+                     double vmin, long ns, const double *psi0, double *out) {
+  const long N = nx * ny * nz;
+  int rc = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long s = 0; s < ns; s++) {
+    int r = or_filter_state(nx, ny, nz, ksqr, potl, an, zn, nc, ms, dE_1, vmin, psi0 + s * N,
+                            out + s * ms * N);
+    if (r) {
+#pragma omp critical
+      rc = r;
+    }
+  }
+  return rc;
+}
+
 /* filter/norm.c:40-54 */
 void or_normalize_all(double *psi, double dv, long m, long ngrid) {
 #pragma omp parallel for schedule(static)

@@ -101,6 +101,10 @@ int or_filter_state(long nx, long ny, long nz, const double *ksqr,
                     const double *potl, const double *an, const double *zn,
                     long nc, long ms, double dE_1, double vmin,
                     const double *psi0, double *out);
+/* ns states in parallel, one per thread; out [ns][ms][N]. */
+int or_filter_states(long nx, long ny, long nz, const double *ksqr, const double *potl,
+                     const double *an, const double *zn, long nc, long ms, double dE_1,
+                     double vmin, long ns, const double *psi0, double *out);
 
 /* ---- norms / energies (filter/norm.c:40-54, filter/energy.c:28-49) ------ */
 void or_normalize_all(double *psi, double dv, long m, long ngrid);

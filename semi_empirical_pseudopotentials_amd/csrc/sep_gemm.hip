@@ -150,25 +150,28 @@ __global__ void __launch_bounds__(256) k_gram_t(const R *__restrict__ A, long ld
   long g = g0;
   const long gfull = g0 + ((g1 - g0) / GKB) * GKB;
   if (g < gfull) {
-    // loads past the last full block re-read the last one (never used):
-    // unconditional loads keep the wait counts uniform, so the MFMAs of
-    // block n wait only for block n's loads
+    // Invariant at the loop head: stage u < ST - 1 holds block g + u GKB.
+    // The body is ST straight-line steps with no branch, so the wait before
+    // the MFMAs of a stage covers only that stage's loads (the compiler
+    // drains every outstanding load at a loop head whose incoming paths
+    // differ).  Loads past the last full block re-read it (never used).
     const long glast = gfull - GKB;
 #pragma unroll
     for (int st = 0; st < ST - 1; st++) load(st, g + st * GKB < gfull ? g + st * GKB : glast);
-    bool done = false;
-    while (!done) {
+    for (; g + ST * GKB <= gfull;) {
 #pragma unroll
       for (int u = 0; u < ST; u++) {
-        if (!done) {
-          const long gl = g + (ST - 1) * GKB;
-          load((u + ST - 1) % ST, gl < gfull ? gl : glast);
-          step(u);
-          g += GKB;
-          done = g >= gfull;
-        }
+        const long gl = g + (ST - 1) * GKB;
+        load((u + ST - 1) % ST, gl < gfull ? gl : glast);
+        step(u);
+        g += GKB;
       }
     }
+    const long rest = (gfull - g) / GKB;  // < ST blocks, in stages 0 .. rest - 1
+#pragma unroll
+    for (int u = 0; u < ST - 1; u++)
+      if (u < rest) step(u);
+    g = gfull;
   }
   if (g < g1) {  // partial last block, element-wise with zero fill
 #pragma unroll
@@ -227,9 +230,13 @@ static GramShape gram_shape(long m, long mb, long K, bool sym, int bw) {
 }
 
 // kernel configuration per precision (wave tile width TJ, pipeline depth
-// ST), from a sweep at m = 512, N = 128^3 on MI355X (TFLOP/s, f64 / f32):
-// TJ 4: ST 2 32-43 / 58-63, ST 3 45 / 66-71, ST 4 44 / 70; TJ 2 (two
-// waves per SIMD): 23-32 / 48-52
+// ST), from sweeps at m = 512, N = 128^3 on MI355X (TFLOP/s, f64 / f32,
+// box-to-box spread ~10 %): TJ 4: ST 2 42 / 58-77, ST 3 45-46 / 66-71, ST 4
+// 44-46 / 57-70; TJ 2 (two waves per SIMD): 23-32 / 48-52.  A register-only
+// probe of v_mfma_f64_16x16x4f64 (no memory traffic) issues at 36-48
+// TFLOP/s on this part (more independent accumulators per wave run slower),
+// while FP64 VALU FMAs reach 65: the f64 Gram at 45 runs at the matrix
+// cores' practical FP64 rate, not at a memory limit.
 template <typename R>
 struct GramCfg;
 template <>
@@ -238,7 +245,7 @@ struct GramCfg<double> {
 };
 template <>
 struct GramCfg<float> {
-  static constexpr int TJ = 4, ST = 3;
+  static constexpr int TJ = 4, ST = 2;
 };
 
 long gram_work_len(long m, long mb, long K) {
@@ -323,21 +330,41 @@ __global__ void __launch_bounds__(256) k_combine_t(const R *__restrict__ A, long
     gin[p][1] = gp[p] + 1 < N;
   }
   const bool full_g = g0 + 16 * TG <= N;
-  long jc[4];
-#pragma unroll
-  for (int x = 0; x < 4; x++) {
-    const long j = j0 + 16 * x + li;
-    jc[x] = j < mo ? j : -1;
-  }
+  // FP32: MFMA row r of j-tile x stands for output state j0 + 4 r + x, so
+  // that a lane's four M entries per contraction index are consecutive (two
+  // 16-byte loads instead of four scattered ones: 47-55 -> 63-72 TFLOP/s);
+  // FP64 keeps j0 + 16 x + r (the permuted form measured 27-34 against
+  // 34-50 TFLOP/s)
+  constexpr bool PERM = sizeof(R) == 4;
+  const long jl = j0 + 4 * li;
+  const bool jfull = PERM && jl + 3 < mo && (ldm & 1) == 0 &&
+                     (reinterpret_cast<uintptr_t>(M) & 15) == 0;
   R mv[ST][4][4], av[ST][TG][4];  // mv[x][s]: M^T(j, i), av[y][s]: A(i, g)
   auto load = [&](int st, long i0) {
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const long i = i0 + 4 * h + s;
       const bool iin = i < m;
+      if (PERM) {
+        const double *mr = M + (iin ? i : 0) * ldm + jl;
+        if (jfull) {
+          const double2 q0 = *reinterpret_cast<const double2 *>(mr);
+          const double2 q1 = *reinterpret_cast<const double2 *>(mr + 2);
+          mv[st][0][s] = iin ? (R)q0.x : (R)0;
+          mv[st][1][s] = iin ? (R)q0.y : (R)0;
+          mv[st][2][s] = iin ? (R)q1.x : (R)0;
+          mv[st][3][s] = iin ? (R)q1.y : (R)0;
+        } else {
 #pragma unroll
-      for (int x = 0; x < 4; x++)
-        mv[st][x][s] = (iin && jc[x] >= 0) ? (R)M[i * ldm + jc[x]] : (R)0;
+          for (int x = 0; x < 4; x++) mv[st][x][s] = (iin && jl + x < mo) ? (R)mr[x] : (R)0;
+        }
+      } else {
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+          const long j = j0 + 16 * x + li;
+          mv[st][x][s] = (iin && j < mo) ? (R)M[i * ldm + j] : (R)0;
+        }
+      }
       const R *ar = A + (iin ? i : 0) * lda;
 #pragma unroll
       for (int p = 0; p < NP; p++) {
@@ -368,29 +395,38 @@ __global__ void __launch_bounds__(256) k_combine_t(const R *__restrict__ A, long
         for (int y = 0; y < TG; y++)
           acc[x][y] = Mfma<R>::op(mv[st][x][s], av[st][y][s], acc[x][y]);
   };
+  // the same branch-free pipeline as the Gram's: at the loop head stage
+  // u < ST - 1 holds i-block i0 + u GKB; blocks past the end re-read block 0
   long i0 = 0;
-  // blocks past the end re-read block 0 (never used): unconditional loads
-  // keep the wait counts uniform
 #pragma unroll
   for (int st = 0; st < ST - 1; st++) load(st, st * GKB < m ? st * GKB : 0);
-  bool done = false;
-  while (!done) {
+  for (; i0 + ST * GKB <= m;) {
 #pragma unroll
     for (int u = 0; u < ST; u++) {
-      if (!done) {
-        const long il = i0 + (ST - 1) * GKB;
-        load((u + ST - 1) % ST, il < m ? il : 0);
-        step(u);
-        i0 += GKB;
-        done = i0 >= m;
-      }
+      const long il = i0 + (ST - 1) * GKB;
+      load((u + ST - 1) % ST, il < m ? il : 0);
+      step(u);
+      i0 += GKB;
+    }
+  }
+  {
+    // the rest: stages u < ST - 1 hold blocks i0 + u GKB (the final block
+    // may be partial, load() zero-fills i >= m); blocks beyond them are
+    // loaded and multiplied one at a time
+    const long rest = (m - i0 + GKB - 1) / GKB;
+#pragma unroll
+    for (int u = 0; u < ST - 1; u++)
+      if (u < rest) step(u);
+    for (long ib = i0 + (ST - 1) * GKB; ib < m; ib += GKB) {
+      load(0, ib);
+      step(0);
     }
   }
 #pragma unroll
   for (int x = 0; x < 4; x++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const long j = j0 + 16 * x + Mfma<R>::row(lane, r);
+      const long j = PERM ? j0 + 4 * Mfma<R>::row(lane, r) + x : j0 + 16 * x + Mfma<R>::row(lane, r);
       if (j >= mo) continue;
       R *op = Out + j * ldo;
 #pragma unroll
@@ -409,8 +445,8 @@ __global__ void __launch_bounds__(256) k_combine_t(const R *__restrict__ A, long
     }
 }
 
-// from the same sweep (TFLOP/s, f64 / f32): TG 4: ST 1 37-50 / 47-55, ST 2
-// 38-41 / 47; TG 2: ST 1-3 40-45 / 37
+// from the same sweeps (TFLOP/s, f64 / f32): TG 4: ST 1 34-50 / 47-55, ST 2
+// 37-41 / 39-47, ST 3 40 / 40; TG 2: ST 1-3 34-45 / 37
 template <typename R>
 struct CombineCfg;
 template <>
