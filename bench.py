@@ -428,6 +428,13 @@ def main():
         tot, ts = sum(per.values()), sum(tms.values())
         out["step"] = {"ms": round(ts, 5), "bytes": tot, "GBs": round(tot / (ts * 1e-3) / 1e9, 1),
                        "frac": round(tot / (ts * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        # the same step against the algorithmic bytes of the three-pass
+        # schedule, N (7c + r) per complex vector and H application: what the
+        # step would move with whole-plane kernels, whatever family runs
+        alg = (7 * c + r) * N * nvec
+        out["step_on_3pass_bytes"] = {"bytes": alg,
+                                      "GBs": round(alg / (ts * 1e-3) / 1e9, 1),
+                                      "frac": round(alg / (ts * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
         return out
 
     r32 = roof(live32, 8, 4)

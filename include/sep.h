@@ -70,6 +70,18 @@ int sep_build_potential(sep_ctx *ctx, long natom, const double *xyz, const int *
                         const double *pot, long stride, const double *dr,
                         const double *expot, double *potl_out, double *vmin, double *vmax);
 
+/* Table interpolation used by sep_build_potential and sep_hf_adjoint on this
+ * context.  SEP_INTERP_LINEAR (default) is the reference's piecewise-linear
+ * interpolate() (filter/interpolate.c:11-23), a function of |r - R| with
+ * kinks at every table row.  SEP_INTERP_CUBIC is an extension with no
+ * counterpart in the reference: the cubic Hermite interpolant of the same
+ * rows (Catmull-Rom tangents, one-sided at the table ends, same segment and
+ * cutoff), C^1 in |r - R|, for derivative checks and fits that need V(r)
+ * smooth in the atom positions; it differs from the linear V by O(dr^2 v''). */
+#define SEP_INTERP_LINEAR 0
+#define SEP_INTERP_CUBIC 1
+int sep_set_interp(sep_ctx *ctx, int interp);
+
 /* ---------------------------------------------------------------- H . psi */
 /* phi <- T phi + V phi on `batch` complex vectors (in != out allowed).
  * Replaces hamiltonian(), filter/hamiltonian.c:11-23 (with kinetic(), :27-44). */

@@ -53,6 +53,7 @@ def lib():
         L.or_newton_coeffs.argtypes = [_L, _L, _dp, _dp, _D, _D, _D, _dp]
         L.or_filter_state.argtypes = [_L, _L, _L, _dp, _dp, _dp, _dp, _L, _L, _D,
                                       _D, _dp, _dp]
+        L.or_set_interp.argtypes = [ctypes.c_int]
         L.or_filter_states.argtypes = [_L, _L, _L, _dp, _dp, _dp, _dp, _L, _L, _D, _D, _L, _dp,
                                        _dp]
         L.or_normalize_all.argtypes = [_dp, _D, _L, _L]
@@ -153,6 +154,11 @@ def filter_state(grid, ksq, potl, an, zn, dE_1, vmin, psi0):
     lib().or_filter_state(nx, ny, nz, _p(_c(ksq)), _p(_c(potl)), _p(an), _p(_c(zn)), nc,
                           ms, dE_1, vmin, _p(_c(psi0)), _p(out))
     return out
+
+
+def set_interp(interp):
+    """0 = linear table interpolation (the reference), 1 = cubic Hermite."""
+    lib().or_set_interp(int(interp))
 
 
 def filter_states(grid, ksq, potl, an, zn, dE_1, vmin, psi0):

@@ -300,10 +300,66 @@ int or_ksqr(long nx, long ny, long nz, double dx, double dy, double dz,
 
 /* filter/interpolate.c:11-23; the scale factor is
  * filterStrainCube/interpolate.c:20-21 (scale 1 reproduces the plain form). */
+/* Table interpolation of the V(r) builder and its adjoint: 0 = the
+ * reference's linear interpolate() (filter/interpolate.c:11-23), 1 = the
+ * cubic Hermite extension (libsep's SEP_INTERP_CUBIC: Catmull-Rom tangents,
+ * one-sided at both table ends, same segment and cutoff).  Written here from
+ * the Hermite basis and the tangents directly. */
+static int g_interp = 0;
+void or_set_interp(int interp) { g_interp = interp; }
+
+/* tangent (per row step) of an n-row table at row k */
+static double herm_tangent(const double *p, long n, long k) {
+  if (k == 0) return p[1] - p[0];
+  if (k == n - 1) return p[n - 1] - p[n - 2];
+  return 0.5 * (p[k + 1] - p[k - 1]);
+}
+/* value and d/dt on segment i at t in [0, 1) */
+static void herm_eval(const double *p, long n, long i, double t, double *val, double *dval) {
+  double m0 = herm_tangent(p, n, i), m1 = herm_tangent(p, n, i + 1);
+  double a = p[i], b = p[i + 1];
+  /* Horner form of h00 a + h10 m0 + h01 b + h11 m1 */
+  double c3 = 2.0 * a + m0 - 2.0 * b + m1;
+  double c2 = -3.0 * a - 2.0 * m0 + 3.0 * b - m1;
+  *val = ((c3 * t + c2) * t + m0) * t + a;
+  *dval = (3.0 * c3 * t + 2.0 * c2) * t + m0;
+}
+/* d(value)/d(p_(i-1+q)), q = 0..3, on segment i at t: the Hermite basis
+ * times the derivative of each tangent with respect to each row (rows
+ * outside the table get weight 0) */
+static void herm_row_weights(long n, long i, double t, double *wq) {
+  double t2 = t * t, t3 = t2 * t;
+  double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + t;
+  double h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
+  for (int q = 0; q < 4; q++) wq[q] = 0.0;
+  wq[1] += h00; /* p_i */
+  wq[2] += h01; /* p_(i+1) */
+  if (i == 0) { /* m0 = p1 - p0 */
+    wq[2] += h10;
+    wq[1] -= h10;
+  } else { /* m0 = (p_(i+1) - p_(i-1)) / 2 */
+    wq[2] += 0.5 * h10;
+    wq[0] -= 0.5 * h10;
+  }
+  if (i + 1 == n - 1) { /* m1 = p_(n-1) - p_(n-2) */
+    wq[2] += h11;
+    wq[1] -= h11;
+  } else { /* m1 = (p_(i+2) - p_i) / 2 */
+    wq[3] += 0.5 * h11;
+    wq[1] -= 0.5 * h11;
+  }
+}
+
 double or_interpolate(double r, double dr, const double *vr, const double *pot,
                       long stride, long n, long j, double scale) {
   long i = (long)(r / dr);
   if (i > (n - 2)) return 0.0;
+  if (g_interp == 1) {
+    const double *p = pot + j * stride, *v = vr + j * stride;
+    double val, dval;
+    herm_eval(p, n, i, (r - v[i]) / (v[i + 1] - v[i]), &val, &dval);
+    return scale * val;
+  }
   const double *p = pot + j * stride, *v = vr + j * stride;
   double a = scale * (p[i + 1] - p[i]) / (v[i + 1] - v[i]);
   double b = scale * p[i] - v[i] * a;
@@ -390,6 +446,22 @@ int or_hf_adjoint(long nx, long ny, long nz, double xmin, double ymin, double zm
           long i = (long)(del / dr[t]);
           if (i > npot[t] - 2) continue;
           double r = rho[nx * (ny * jz + jy) + jx];
+          if (g_interp == 1) {
+            double tt = (del - v[i]) / (v[i + 1] - v[i]), val, dval, wq[4];
+            herm_eval(p, npot[t], i, tt, &val, &dval);
+            dval /= (v[i + 1] - v[i]);
+            if (del > 0.0) {
+              double f = -r * s * dval / del;
+              gx += f * ddx;
+              gy += f * ddy;
+              gz += f * ddz;
+            }
+            gs += r * val;
+            herm_row_weights(npot[t], i, tt, wq);
+            for (int q = 0; q < 4; q++)
+              if (wq[q] != 0.0) h[i - 1 + q] += r * s * wq[q];
+            continue;
+          }
           double slope = (p[i + 1] - p[i]) / (v[i + 1] - v[i]);
           double w = (del - v[i]) / (v[i + 1] - v[i]);
           if (del > 0.0) {

@@ -44,6 +44,11 @@ int or_fft3d(long nx, long ny, long nz, double *data, int sign);
 /* 1D transform of one contiguous line (for tests). */
 int or_fft1d(long n, double *data, int sign);
 
+/* Table interpolation of or_interpolate / or_build_potential /
+ * or_hf_adjoint: 0 = linear (the reference), 1 = cubic Hermite (the smooth
+ * extension libsep calls SEP_INTERP_CUBIC).  Process-wide. */
+void or_set_interp(int interp);
+
 /* ---- grid / operator setup (filter/init.c) ------------------------------ */
 /* ksqr exactly as filter/init.c:125-144 (1/N and the Ekinmax clamp folded). */
 int or_ksqr(long nx, long ny, long nz, double dx, double dy, double dz,

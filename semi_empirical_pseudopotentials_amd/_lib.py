@@ -15,6 +15,8 @@ FILTER_BIN = os.path.join(_HERE, "bin", "filter")
 COUPLINGS_BIN = os.path.join(_HERE, "bin", "couplings")
 
 SEP_OK = 0
+SEP_INTERP_LINEAR = 0
+SEP_INTERP_CUBIC = 1
 SEP_PREC_FP64 = 0
 SEP_PREC_FP32 = 1
 _ERRNAMES = {1: "SEP_ERR_ARG", 2: "SEP_ERR_UNSUPPORTED", 3: "SEP_ERR_HIP", 4: "SEP_ERR_IO",
@@ -71,6 +73,7 @@ SIGNATURES = {
     "sep_hnorm": (_I, [_V, _dp, _dp, _L, _D, _D, _D]),
     "sep_apply_h_dev": (_I, [_V, _V, _V, _L, _I]),
     "sep_apply_h_dev_noalloc": (_I, [_V, _V, _V, _L, _I]),
+    "sep_set_interp": (_I, [_V, _I]),
     "sep_energy_range": (_I, [_V, _D, _dp, _dp, _lp]),
     "sep_rng_reset": (None, [ctypes.POINTER(Rng)]),
     "sep_ran_nrc": (_D, [ctypes.POINTER(Rng), _lp]),
@@ -291,6 +294,11 @@ class Context:
         assert potl.size == self.N
         self._chk(lib().sep_set_potential(self._h, _p(potl)), "sep_set_potential")
         self.vmin, self.vmax = float(potl.min()), float(potl.max())
+
+    def set_interp(self, interp):
+        """SEP_INTERP_LINEAR (reference) or SEP_INTERP_CUBIC table interpolation for
+        build_potential and hf_adjoint on this context."""
+        self._chk(lib().sep_set_interp(self._h, int(interp)), "sep_set_interp")
 
     def build_potential(self, xyz, types, npot, vr, pot, dr, scale=None, expot=None):
         xyz = _f64(xyz).reshape(-1, 3)

@@ -27,7 +27,8 @@ import warnings
 import numpy as np
 import torch
 
-from ._lib import SEP_PREC_FP64, Context, init_psi, leja_points, newton_coeffs
+from ._lib import (SEP_INTERP_CUBIC, SEP_INTERP_LINEAR, SEP_PREC_FP64, Context, init_psi,
+                   leja_points, newton_coeffs)
 
 
 def radial_tables(params, rmax=12.0, dr=0.01):
@@ -64,10 +65,15 @@ class Problem:
 
     species_of_atom: species index per atom (rows of params); grid: (nx, ny,
     nz); box: (dx, dy, dz, xmin, ymin, zmin) in bohr; targets: energies in Ha.
+    interp: "linear" builds V(r) with the reference's piecewise-linear table
+    interpolation (filter/interpolate.c:11-23), whose kinks at every table
+    row make E piecewise smooth in the atom positions; "cubic" uses the C^1
+    cubic Hermite interpolant of the same tables (SEP_INTERP_CUBIC), which
+    makes the position derivatives agree with finite differences.
     """
 
     def __init__(self, species_of_atom, grid, box, nc, ns, targets, seed=-1001,
-                 rmax=12.0, dr=0.01, sigma_gate=1e-6, cutoff=1e-10):
+                 rmax=12.0, dr=0.01, sigma_gate=1e-6, cutoff=1e-10, interp="linear"):
         self.types = np.ascontiguousarray(species_of_atom, dtype=np.int32)
         self.grid = tuple(int(g) for g in grid)
         self.box = tuple(float(b) for b in box)
@@ -78,6 +84,9 @@ class Problem:
         nx, ny, nz = self.grid
         dx, dy, dz, x0, y0, z0 = self.box
         self.ctx = Context(nx, ny, nz, dx, dy, dz, x0, y0, z0)
+        if interp not in ("linear", "cubic"):
+            raise ValueError(f"interp must be 'linear' or 'cubic', not {interp!r}")
+        self.ctx.set_interp(SEP_INTERP_CUBIC if interp == "cubic" else SEP_INTERP_LINEAR)
         self.dv = dx * dy * dz
         self.generation = 0
         self._psi0 = None

@@ -13,9 +13,11 @@
 // With rho_n = psi_n^2 these are dE_n/d(.) for a converged eigenpair
 // (Hellmann-Feynman); with i != j the couplings <psi_i|dV/dR_a|psi_j> of
 // nonAdiabaticCoupling/nonadiabatic.c:143-301.  The derivative is that of the
-// piecewise-linear forward model: segment k = (long)(del / dr_t), zero
-// beyond row npot_t - 2, interp = (1 - w) pot[k] + w pot[k+1] with
-// w = (del - vr[k]) / (vr[k+1] - vr[k]).
+// forward model: segment k = (long)(del / dr_t), zero beyond row npot_t - 2,
+// linear interp = (1 - w) pot[k] + w pot[k+1] with w = (del - vr[k]) /
+// (vr[k+1] - vr[k]), or (SEP_INTERP_CUBIC) the cubic Hermite interpolant of
+// cubic_weights()/cubic_rows() on rows k-1 .. k+2, whose derivative in del
+// is continuous.
 //
 // Work split: one block per (atom, slice of the atom's bounding box of grid
 // points within the table range); each block reduces its partial sums over
@@ -36,7 +38,7 @@ __global__ void __launch_bounds__(HF_THREADS) k_hf_atom(
     double dz, const double *__restrict__ xyz, const int *__restrict__ type,
     const double *__restrict__ scale, const long *__restrict__ npot, const double *__restrict__ vr,
     const double *__restrict__ pot, long stride, const double *__restrict__ dr,
-    const double *__restrict__ rho, long N, int np, int p0, int ntype, int nsplit,
+    const double *__restrict__ rho, long N, int np, int p0, int ntype, int nsplit, int interp,
     double *__restrict__ work, double *__restrict__ htab) {
   extern __shared__ double hist[];  // [PB][npot_t]
   const int a = blockIdx.x, split = blockIdx.y;
@@ -74,8 +76,27 @@ __global__ void __launch_bounds__(HF_THREADS) k_hf_atom(
     if (k > nt - 2) continue;
     const double span = vt[k + 1] - vt[k];
     const double w = (del - vt[k]) / span;
-    const double val = (1.0 - w) * pt[k] + w * pt[k + 1];
-    const double g = del > 0.0 ? -sa * (pt[k + 1] - pt[k]) / span / del : 0.0;
+    // value, d(value)/d(del) without the scale, and the table-row weights
+    long rows[4] = {k, k + 1, k + 1, k + 1};
+    double wr[4] = {1.0 - w, w, 0.0, 0.0};
+    double val, dval;
+    if (interp == SEP_INTERP_CUBIC) {
+      double cw[4], cd[4], dr4[4];
+      cubic_weights(w, cw, cd);
+      cubic_rows(k, nt, cw, rows, wr);
+      cubic_rows(k, nt, cd, rows, dr4);
+      val = dval = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        val += wr[c] * pt[rows[c]];
+        dval += dr4[c] * pt[rows[c]];
+      }
+      dval /= span;
+    } else {
+      val = (1.0 - w) * pt[k] + w * pt[k + 1];
+      dval = (pt[k + 1] - pt[k]) / span;
+    }
+    const double g = del > 0.0 ? -sa * dval / del : 0.0;
     const long gi = ((long)iz * ny + iy) * nx + ix;
 #pragma unroll
     for (int q = 0; q < PB; q++) {
@@ -85,8 +106,9 @@ __global__ void __launch_bounds__(HF_THREADS) k_hf_atom(
       acc[q][1] += r * g * ddy;
       acc[q][2] += r * g * ddz;
       acc[q][3] += r * val;
-      atomicAdd(&hist[q * nt + k], r * sa * (1.0 - w));
-      atomicAdd(&hist[q * nt + k + 1], r * sa * w);
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (wr[c] != 0.0) atomicAdd(&hist[q * nt + rows[c]], r * sa * wr[c]);
     }
   }
   // block reduction of the partial sums (wave shuffles, then LDS across waves)
@@ -162,7 +184,7 @@ static int hf_pass(const GridDev &gd, double xmin, double ymin, double zmin, dou
                    double dz, double dv, int natom, const double *xyz, const int *type,
                    const double *scale, const long *npot, long maxnpot, const double *vr,
                    const double *pot, long stride, const double *dr, int ntype, const double *rho,
-                   int np, int p0, int nsplit, double *work, double *htab, double *dR,
+                   int np, int p0, int nsplit, int interp, double *work, double *htab, double *dR,
                    double *dscale, hipStream_t st) {
   const size_t lds = sizeof(double) * PB * (size_t)maxnpot;
   static bool attr = false;
@@ -173,7 +195,7 @@ static int hf_pass(const GridDev &gd, double xmin, double ymin, double zmin, dou
   }
   hipLaunchKernelGGL(k_hf_atom<PB>, dim3((unsigned)natom, (unsigned)nsplit), dim3(HF_THREADS), lds,
                      st, gd.nx, gd.ny, gd.nz, xmin, ymin, zmin, dx, dy, dz, xyz, type, scale, npot,
-                     vr, pot, stride, dr, rho, gd.N, np, p0, ntype, nsplit, work, htab);
+                     vr, pot, stride, dr, rho, gd.N, np, p0, ntype, nsplit, interp, work, htab);
   const long nthr = (long)natom * PB;
   hipLaunchKernelGGL(k_hf_reduce<PB>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, work,
                      natom, nsplit, np, p0, dv, dR, dscale);
@@ -192,8 +214,8 @@ int hf_adjoint_dev(const GridDev &gd, double xmin, double ymin, double zmin, dou
                    double dz, double dv, int natom, const double *xyz, const int *type,
                    const double *scale, const long *npot, long maxnpot, const double *vr,
                    const double *pot, long stride, const double *dr, int ntype, const double *rho,
-                   int np, int nsplit, double *work, double *htab, double *dR, double *dscale,
-                   hipStream_t st) {
+                   int np, int nsplit, int interp, double *work, double *htab, double *dR,
+                   double *dscale, hipStream_t st) {
   const int pb = hf_pairs_per_pass(maxnpot);
   if (pb == 0) return SEP_ERR_ARG;
   for (int p0 = 0; p0 < np; p0 += pb) {
@@ -201,23 +223,23 @@ int hf_adjoint_dev(const GridDev &gd, double xmin, double ymin, double zmin, dou
     switch (pb) {
       case 8:
         rc = hf_pass<8>(gd, xmin, ymin, zmin, dx, dy, dz, dv, natom, xyz, type, scale, npot,
-                        maxnpot, vr, pot, stride, dr, ntype, rho, np, p0, nsplit, work, htab, dR,
-                        dscale, st);
+                        maxnpot, vr, pot, stride, dr, ntype, rho, np, p0, nsplit, interp, work, htab,
+                        dR, dscale, st);
         break;
       case 4:
         rc = hf_pass<4>(gd, xmin, ymin, zmin, dx, dy, dz, dv, natom, xyz, type, scale, npot,
-                        maxnpot, vr, pot, stride, dr, ntype, rho, np, p0, nsplit, work, htab, dR,
-                        dscale, st);
+                        maxnpot, vr, pot, stride, dr, ntype, rho, np, p0, nsplit, interp, work, htab,
+                        dR, dscale, st);
         break;
       case 2:
         rc = hf_pass<2>(gd, xmin, ymin, zmin, dx, dy, dz, dv, natom, xyz, type, scale, npot,
-                        maxnpot, vr, pot, stride, dr, ntype, rho, np, p0, nsplit, work, htab, dR,
-                        dscale, st);
+                        maxnpot, vr, pot, stride, dr, ntype, rho, np, p0, nsplit, interp, work, htab,
+                        dR, dscale, st);
         break;
       default:
         rc = hf_pass<1>(gd, xmin, ymin, zmin, dx, dy, dz, dv, natom, xyz, type, scale, npot,
-                        maxnpot, vr, pot, stride, dr, ntype, rho, np, p0, nsplit, work, htab, dR,
-                        dscale, st);
+                        maxnpot, vr, pot, stride, dr, ntype, rho, np, p0, nsplit, interp, work, htab,
+                        dR, dscale, st);
     }
     if (rc != SEP_OK) return rc;
   }

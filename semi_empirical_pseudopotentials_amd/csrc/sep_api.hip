@@ -35,6 +35,8 @@ struct sep_ctx {
   size_t tb_bytes = 0;
   double *scr = nullptr;
   size_t scr_len = 0;
+  // radial-table interpolation of the V(r) builder and its adjoint
+  int interp = SEP_INTERP_LINEAR;
   // split-K partial sums of the matrix-core Gram (gram_dev)
   double *gwork = nullptr;
   size_t gwork_bytes = 0;
@@ -345,7 +347,7 @@ int sep_build_potential(sep_ctx *c, long natom, const double *xyz, const int *ty
   int rc = build_potential_dev(c->gd, c->xmin, c->ymin, c->zmin, c->dx, c->dy, c->dz, natom,
                                xyz_d.as<double>(), type_d.as<int>(), scale_d.as<double>(),
                                npot_d.as<long>(), vr_d.as<double>(), pot_d.as<double>(), stride,
-                               dr_d.as<double>(), ex_e, ex_x0, ex_t, c->pot64, c->st);
+                               dr_d.as<double>(), ex_e, ex_x0, ex_t, c->interp, c->pot64, c->st);
   HIPCHK(c, hipStreamSynchronize(c->st));
   if (rc != SEP_OK) return fail(c, rc, "potential kernel launch failed");
   RCHK(finish_potential(c));
@@ -1474,7 +1476,7 @@ extern "C" int sep_hf_adjoint(sep_ctx *c, long npairs, const long *pairs, const 
                           xyz_d.as<double>(), type_d.as<int>(), scale_d.as<double>(),
                           npot_d.as<long>(), maxnpot, vr_d.as<double>(), pot_d.as<double>(), stride,
                           dr_d.as<double>(), ntype, rho_d.as<double>(), (int)np, (int)nsplit,
-                          work_d.as<double>(), htab_d.as<double>(), dR_d.as<double>(),
+                          c->interp, work_d.as<double>(), htab_d.as<double>(), dR_d.as<double>(),
                           ds_d.as<double>(), c->st);
   if (rc != SEP_OK) return fail(c, rc, "adjoint kernel launch failed");
   if (dR)
@@ -1728,6 +1730,12 @@ extern "C" int sep_trim(sep_ctx *c) {
   if (c->gwork) HIPCHK(c, hipFree(c->gwork));
   c->gwork = nullptr;
   c->gwork_bytes = 0;
+  return SEP_OK;
+}
+
+extern "C" int sep_set_interp(sep_ctx *c, int interp) {
+  if (!c || (interp != SEP_INTERP_LINEAR && interp != SEP_INTERP_CUBIC)) return SEP_ERR_ARG;
+  c->interp = interp;
   return SEP_OK;
 }
 

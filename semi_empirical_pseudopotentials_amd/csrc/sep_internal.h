@@ -49,12 +49,69 @@ int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx
               const R *pot, int nvec, double alpha, double beta, bool fwd_next, hipStream_t st,
               hipEvent_t *ev = nullptr);
 
+// ---- cubic Hermite interpolation of a uniform radial table (the smooth
+// option of the differentiable path, SEP_INTERP_CUBIC).  On segment i
+// (r_i <= r < r_(i+1), t = (r - r_i) / (r_(i+1) - r_i)) the value is
+//   h00(t) p_i + h10(t) m_i + h01(t) p_(i+1) + h11(t) m_(i+1)
+// with the Catmull-Rom tangents m_i = (p_(i+1) - p_(i-1)) / 2 (one-sided,
+// p_1 - p_0 and p_(n-1) - p_(n-2), at the table ends), which is C^1 in r and
+// reproduces quadratics exactly away from the ends.  As weights on the four
+// table rows i-1 .. i+2: w[0] = -h10/2, w[1] = h00 - h11/2, w[2] = h01 +
+// h10/2, w[3] = h11/2 (rows outside the table clamp to the end rows and
+// cubic_edge() adds the one-sided corrections); dw = dw/dt.
+__host__ __device__ inline void cubic_weights(double t, double *w, double *dw) {
+  const double t2 = t * t, t3 = t2 * t;
+  const double h00 = 2.0 * t3 - 3.0 * t2 + 1.0, h10 = t3 - 2.0 * t2 + t;
+  const double h01 = -2.0 * t3 + 3.0 * t2, h11 = t3 - t2;
+  const double d00 = 6.0 * t2 - 6.0 * t, d10 = 3.0 * t2 - 4.0 * t + 1.0;
+  const double d01 = -6.0 * t2 + 6.0 * t, d11 = 3.0 * t2 - 2.0 * t;
+  w[0] = -0.5 * h10;
+  w[1] = h00 - 0.5 * h11;
+  w[2] = h01 + 0.5 * h10;
+  w[3] = 0.5 * h11;
+  dw[0] = -0.5 * d10;
+  dw[1] = d00 - 0.5 * d11;
+  dw[2] = d01 + 0.5 * d10;
+  dw[3] = 0.5 * d11;
+}
+// Row weights of segment i of an n-row table after the end corrections:
+// at i = 0 the tangent m_0 = p_1 - p_0 (not (p_1 - p_(-1)) / 2), at
+// i + 1 = n - 1 the tangent m_(n-1) = p_(n-1) - p_(n-2).  Fills rows[4]
+// (table indices) and wr[4] (weights; w or dw from cubic_weights).
+__host__ __device__ inline void cubic_rows(long i, long n, const double *w, long *rows,
+                                           double *wr) {
+  // central-tangent weights on rows i-1, i, i+1, i+2
+  rows[0] = i - 1, rows[1] = i, rows[2] = i + 1, rows[3] = i + 2;
+  wr[0] = w[0], wr[1] = w[1], wr[2] = w[2], wr[3] = w[3];
+  if (i == 0) {  // m_0 = p_1 - p_0 = 2 * (central form with p_-1 := 2 p_0 - p_1) / 2
+    // w0 * p_-1 with p_-1 = 2 p_0 - p_1
+    wr[1] += 2.0 * wr[0];
+    wr[2] -= wr[0];
+    wr[0] = 0.0;
+    rows[0] = 0;
+  }
+  if (i + 2 > n - 1) {  // p_(n) := 2 p_(n-1) - p_(n-2)
+    wr[2] += 2.0 * wr[3];
+    wr[1] -= wr[3];
+    wr[3] = 0.0;
+    rows[3] = n - 1;
+  }
+}
+// The value correction that cubic_rows applies at the table ends, for the
+// forward kernel's clamped-index form (0 away from the ends).
+__host__ __device__ inline double cubic_edge(long i, long n, const double *w, const double *p) {
+  double c = 0.0;
+  if (i == 0) c += w[0] * (p[0] - p[1]);               // p_-1 = 2 p_0 - p_1 vs p_0
+  if (i + 2 > n - 1) c += w[3] * (p[n - 1] - p[n - 2]);  // p_n = 2 p_(n-1) - p_(n-2) vs p_(n-1)
+  return c;
+}
+
 // ---- element-wise / reduction kernels (sep_kernels.hip)
 int build_potential_dev(const GridDev &gd, double xmin, double ymin, double zmin, double dx,
                         double dy, double dz, long natom, const double *xyz_d, const int *type_d,
                         const double *scale_d, const long *npot_d, const double *vr_d,
                         const double *pot_d, long stride, const double *dr_d, double ex_e,
-                        double ex_x0, double ex_t, double *potl_d, hipStream_t st);
+                        double ex_x0, double ex_t, int interp, double *potl_d, hipStream_t st);
 // out[i] = sum_g a[i*lda + g] * b[i*ldb + g] for i < m (deterministic).
 int dots_dev(const double *a, long lda, const double *b, long ldb, long m, long n, double *out_d,
              double *scratch, long scratch_len, hipStream_t st);
@@ -69,8 +126,8 @@ int hf_adjoint_dev(const GridDev &gd, double xmin, double ymin, double zmin, dou
                    double dz, double dv, int natom, const double *xyz, const int *type,
                    const double *scale, const long *npot, long maxnpot, const double *vr,
                    const double *pot, long stride, const double *dr, int ntype, const double *rho,
-                   int np, int nsplit, double *work, double *htab, double *dR, double *dscale,
-                   hipStream_t st);
+                   int np, int nsplit, int interp, double *work, double *htab, double *dR,
+                   double *dscale, hipStream_t st);
 
 
 int sub_scaled_rows_dev(double *y, long ldy, const double *x, long ldx, long m, long n,

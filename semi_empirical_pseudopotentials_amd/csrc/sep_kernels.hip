@@ -11,13 +11,15 @@ namespace sep {
 // filter/init.c:161-182 (sum over atoms in file order, + expot), interpolate
 // of filter/interpolate.c:11-23 with the per-atom scale of
 // filterStrainCube/interpolate.c:20-21.  Contraction is off so each term is
-// rounded exactly as the reference source writes it.
+// rounded exactly as the reference source writes it.  interp = SEP_INTERP_CUBIC
+// replaces the linear segment by the cubic Hermite interpolant of
+// cubic_weights() (same segment, same cutoff), a C^1 function of |r - R|.
 __global__ void __launch_bounds__(256) k_build_potential(
     int nx, int ny, int nz, double xmin, double ymin, double zmin, double dx, double dy, double dz,
     long natom, const double *__restrict__ xyz, const int *__restrict__ type,
     const double *__restrict__ scale, const long *__restrict__ npot, const double *__restrict__ vr,
     const double *__restrict__ pot, long stride, const double *__restrict__ dr, double ex_e,
-    double ex_x0, double ex_t, double *__restrict__ potl) {
+    double ex_x0, double ex_t, int interp, double *__restrict__ potl) {
 #pragma clang fp contract(off)
   const long N = (long)nx * ny * nz;
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -56,6 +58,14 @@ __global__ void __launch_bounds__(256) k_build_potential(
         long i = (long)(del / drt);
         if (i > npot[t] - 2) continue;  // adds 0.0 in the reference
         const double *p = pot + t * stride, *v = vr + t * stride;
+        if (interp == SEP_INTERP_CUBIC) {
+          double w[4], dw[4];
+          cubic_weights((del - v[i]) / (v[i + 1] - v[i]), w, dw);
+          const long n = npot[t];
+          sum += ss[k] * (w[0] * p[i > 0 ? i - 1 : 0] + w[1] * p[i] + w[2] * p[i + 1] +
+                          w[3] * p[i + 2 < n ? i + 2 : n - 1] + cubic_edge(i, n, w, p));
+          continue;
+        }
         double aa = ss[k] * (p[i + 1] - p[i]) / (v[i + 1] - v[i]);
         double bb = ss[k] * p[i] - v[i] * aa;
         sum += aa * del + bb;
@@ -73,11 +83,11 @@ int build_potential_dev(const GridDev &gd, double xmin, double ymin, double zmin
                         double dy, double dz, long natom, const double *xyz_d, const int *type_d,
                         const double *scale_d, const long *npot_d, const double *vr_d,
                         const double *pot_d, long stride, const double *dr_d, double ex_e,
-                        double ex_x0, double ex_t, double *potl_d, hipStream_t st) {
+                        double ex_x0, double ex_t, int interp, double *potl_d, hipStream_t st) {
   dim3 grid((unsigned)((gd.N + 255) / 256));
   hipLaunchKernelGGL(k_build_potential, grid, dim3(256), 0, st, gd.nx, gd.ny, gd.nz, xmin, ymin,
                      zmin, dx, dy, dz, natom, xyz_d, type_d, scale_d, npot_d, vr_d, pot_d, stride,
-                     dr_d, ex_e, ex_x0, ex_t, potl_d);
+                     dr_d, ex_e, ex_x0, ex_t, interp, potl_d);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 
