@@ -248,6 +248,27 @@ int sep_set_filter_lanes(sep_ctx *ctx, int lanes);
  * filter/filter.c:22-23).  Uses the
  * context's state and ring buffers as scratch: call it before filtering. */
 int sep_autotune(sep_ctx *ctx, int precision, long nvec, double *ms_plane, double *ms_lines);
+/* H.psi kernel families of sep_set_hpsi_family. */
+#define SEP_HPSI_PLANE 0         /* whole-plane kernels (passes A and C, C fused with A) */
+#define SEP_HPSI_PLANE_CHUNKED 1 /* same, the fused kernel transposing through half a \
+                                    plane of LDS so that two workgroups share a CU */
+#define SEP_HPSI_LINES 2         /* row + strided-line kernels */
+#define SEP_HPSI_NFAMILY 3
+/* Selects the H.psi kernel family of this context in `precision`.
+ * SEP_ERR_UNSUPPORTED if the grid has no such form (plane families need
+ * nx == ny and a plane length; the chunked form exists for complex64
+ * 128 x 128 planes).  All families give the same H.psi to rounding. */
+int sep_set_hpsi_family(sep_ctx *ctx, int precision, int family);
+int sep_get_hpsi_family(sep_ctx *ctx, int precision, int *family);
+/* Autotuner over every family this grid has and over the number of filter
+ * lanes (1 .. SEP_MAX_LANES, see sep_set_filter_lanes): times the filter
+ * loop's Newton step (H.psi in split form plus the lane split) on nvec
+ * resident random vectors, keeps the fastest family and lane count, and
+ * returns the time per family (ms[SEP_HPSI_NFAMILY], -1 where the grid has
+ * no such form; may be NULL), the chosen family and lane count.  Uses the
+ * context's state and ring buffers as scratch: call it before filtering.
+ * No counterpart in the reference (FFTW_ESTIMATE plans, filter/filter.c:22-23). */
+int sep_autotune_ex(sep_ctx *ctx, int precision, long nvec, double *ms, int *family, int *lanes);
 int sep_get_timing(sep_ctx *ctx, double *avg_ms, long *samples);
 
 /* -------------------------------------------------- resident state block */

@@ -17,6 +17,10 @@ COUPLINGS_BIN = os.path.join(_HERE, "bin", "couplings")
 SEP_OK = 0
 SEP_INTERP_LINEAR = 0
 SEP_INTERP_CUBIC = 1
+SEP_HPSI_PLANE = 0
+SEP_HPSI_PLANE_CHUNKED = 1
+SEP_HPSI_LINES = 2
+SEP_HPSI_NFAMILY = 3
 SEP_PREC_FP64 = 0
 SEP_PREC_FP32 = 1
 _ERRNAMES = {1: "SEP_ERR_ARG", 2: "SEP_ERR_UNSUPPORTED", 3: "SEP_ERR_HIP", 4: "SEP_ERR_IO",
@@ -97,6 +101,9 @@ SIGNATURES = {
     "sep_bench_copy": (_I, [_V, _L, _L, _dp]),
     "sep_set_timing": (_I, [_V, _I]),
     "sep_set_filter_lanes": (_I, [_V, _I]),
+    "sep_set_hpsi_family": (_I, [_V, _I, _I]),
+    "sep_get_hpsi_family": (_I, [_V, _I, _ip]),
+    "sep_autotune_ex": (_I, [_V, _I, _L, _dp, _ip, _ip]),
     "sep_trim": (_I, [_V]),
     "sep_autotune": (_I, [_V, _I, _L, _dp, _dp]),
     "sep_filter_batch": (_I, [_I, _V, _V, _lp, _V, _V, _lp, _lp, _dp, _dp, _I, _I]),
@@ -390,6 +397,25 @@ class Context:
         tp, tl = np.zeros(1), np.zeros(1)
         self._chk(lib().sep_autotune(self._h, precision, nvec, _p(tp), _p(tl)), "sep_autotune")
         return float(tp[0]), float(tl[0])
+
+    def set_hpsi_family(self, precision, family):
+        """H.psi kernel family (SEP_HPSI_PLANE / _PLANE_CHUNKED / _LINES) for this grid."""
+        self._chk(lib().sep_set_hpsi_family(self._h, precision, family), "sep_set_hpsi_family")
+
+    def hpsi_family(self, precision):
+        f = ctypes.c_int(0)
+        self._chk(lib().sep_get_hpsi_family(self._h, precision, ctypes.byref(f)),
+                  "sep_get_hpsi_family")
+        return f.value
+
+    def autotune_ex(self, precision, nvec=8):
+        """Fastest (family, lanes) by short timed filters (sep_autotune_ex);
+        returns (ms per Newton step per family, -1 where absent), family, lanes."""
+        ms = np.zeros(SEP_HPSI_NFAMILY)
+        fam, lanes = ctypes.c_int(0), ctypes.c_int(0)
+        self._chk(lib().sep_autotune_ex(self._h, precision, nvec, _p(ms), ctypes.byref(fam),
+                                        ctypes.byref(lanes)), "sep_autotune_ex")
+        return ms, fam.value, lanes.value
 
     def set_timing(self, on):
         """Sample kernel durations inside the filter loop (see sep_set_timing)."""

@@ -1709,6 +1709,126 @@ extern "C" int sep_autotune(sep_ctx *c, int precision, long nvec, double *ms_pla
   return SEP_OK;
 }
 
+// One short filter (nc Newton steps of 2 nvec random states, one target) on
+// the context's buffers, timed with events on its stream: the autotuner's
+// measure, which sees the lane split and the ring layout of a real filter.
+template <typename R>
+static int tune_filter_ms(sep_ctx *c, long nvec, long nc, double *ms_out) {
+  const long ns = 2 * nvec, N = c->gd.N;
+  RCHK(ensure_states(c, ns, 2));
+  hipLaunchKernelGGL(k_fill_random, dim3(4096), dim3(256), 0, c->st, c->states2, ns * N, 11UL);
+  HIPCHK(c, hipGetLastError());
+  // values immaterial to the timing; kept bounded: |beta| <= 3, dE_1 = 1e-3
+  std::vector<double> an((size_t)nc), zn((size_t)nc);
+  for (long j = 0; j < nc; j++) {
+    an[j] = 1.0 / (double)(j + 1);
+    zn[j] = std::cos(3.141592653589793 * (double)j / (double)nc);
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return fail(c, SEP_ERR_HIP, "autotune: event creation failed");
+  }
+  int rc = SEP_OK;
+  float best = -1.0f;
+  for (int it = 0; it < 3 && rc == SEP_OK; it++) {  // it = 0: warm-up
+    hipEventRecord(e0, c->st);
+    rc = filter_impl<R>(c, c->states2, true, ns, an.data(), zn.data(), nc, 1, -2.0, 4000.0);
+    hipEventRecord(e1, c->st);
+    if (rc == SEP_OK && hipEventSynchronize(e1) != hipSuccess) rc = SEP_ERR_HIP;
+    float t = 0.0f;
+    if (rc == SEP_OK && it > 0 && hipEventElapsedTime(&t, e0, e1) == hipSuccess)
+      best = best < 0.0f || t < best ? t : best;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != SEP_OK) return rc;
+  *ms_out = (double)best;
+  return SEP_OK;
+}
+
+extern "C" int sep_autotune_ex(sep_ctx *c, int precision, long nvec, double *ms, int *family,
+                               int *lanes) {
+  if (!c || nvec < 1 || (precision != SEP_PREC_FP64 && precision != SEP_PREC_FP32))
+    return SEP_ERR_ARG;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  const bool f_lines = c->gd.force_lines[precision], f_chunk = c->gd.fused_chunked[precision];
+  const int l0 = c->lanes;
+  const bool timing = c->timing;
+  c->timing = false;  // live timing turns lanes off
+  const long nc = 24;
+  double t_fam[SEP_HPSI_NFAMILY];
+  double best = -1.0;
+  int bf = -1, bl = 1, rc = SEP_OK;
+  for (int f = 0; f < SEP_HPSI_NFAMILY && rc == SEP_OK; f++) {
+    t_fam[f] = -1.0;
+    if (sep_set_hpsi_family(c, precision, f) != SEP_OK) {
+      c->err.clear();  // a family this grid does not have is no error here
+      continue;
+    }
+    for (int L = 1; L <= SEP_MAX_LANES && L <= nvec && rc == SEP_OK; L *= 2) {
+      c->lanes = L;
+      double t = -1.0;
+      rc = precision == SEP_PREC_FP64 ? tune_filter_ms<double>(c, nvec, nc, &t)
+                                      : tune_filter_ms<float>(c, nvec, nc, &t);
+      if (rc != SEP_OK || t <= 0.0) continue;
+      if (t_fam[f] < 0.0 || t < t_fam[f]) t_fam[f] = t;
+      if (best < 0.0 || t < best) {
+        best = t;
+        bf = f;
+        bl = L;
+      }
+    }
+  }
+  c->timing = timing;
+  if (rc != SEP_OK || bf < 0) {
+    c->gd.force_lines[precision] = f_lines;
+    c->gd.fused_chunked[precision] = f_chunk;
+    c->lanes = l0;
+    return rc != SEP_OK ? rc : fail(c, SEP_ERR_UNSUPPORTED, "autotune: no family ran");
+  }
+  sep_set_hpsi_family(c, precision, bf);
+  c->lanes = bl;
+  if (ms)
+    for (int f = 0; f < SEP_HPSI_NFAMILY; f++) ms[f] = t_fam[f] > 0.0 ? t_fam[f] / nc : -1.0;
+  if (family) *family = bf;
+  if (lanes) *lanes = bl;
+  return SEP_OK;
+}
+
+extern "C" int sep_set_hpsi_family(sep_ctx *c, int precision, int family) {
+  if (!c || (precision != SEP_PREC_FP64 && precision != SEP_PREC_FP32)) return SEP_ERR_ARG;
+  switch (family) {
+    case SEP_HPSI_PLANE:
+    case SEP_HPSI_PLANE_CHUNKED:
+      if (!plane_supported(c->gd.nx, c->gd.ny, precision))
+        return fail(c, SEP_ERR_UNSUPPORTED, "no plane kernels for %d x %d", c->gd.nx, c->gd.ny);
+      if (family == SEP_HPSI_PLANE_CHUNKED && !plane_fused_chunked_supported(c->gd.nx, precision))
+        return fail(c, SEP_ERR_UNSUPPORTED, "no chunked fused kernel for %d x %d in this precision",
+                    c->gd.nx, c->gd.ny);
+      c->gd.force_lines[precision] = false;
+      c->gd.fused_chunked[precision] = family == SEP_HPSI_PLANE_CHUNKED;
+      return SEP_OK;
+    case SEP_HPSI_LINES:
+      c->gd.force_lines[precision] = true;
+      c->gd.fused_chunked[precision] = false;
+      return SEP_OK;
+  }
+  return fail(c, SEP_ERR_ARG, "unknown H.psi family %d", family);
+}
+
+extern "C" int sep_get_hpsi_family(sep_ctx *c, int precision, int *family) {
+  if (!c || !family || (precision != SEP_PREC_FP64 && precision != SEP_PREC_FP32))
+    return SEP_ERR_ARG;
+  const bool plane = !c->gd.force_lines[precision] &&
+                     plane_supported(c->gd.nx, c->gd.ny, precision);
+  *family = !plane ? SEP_HPSI_LINES
+                   : (c->gd.fused_chunked[precision] ? SEP_HPSI_PLANE_CHUNKED : SEP_HPSI_PLANE);
+  return SEP_OK;
+}
+
 extern "C" int sep_set_timing(sep_ctx *c, int on) {
   if (!c) return SEP_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));

@@ -916,6 +916,179 @@ __global__ void __launch_bounds__(T) k_plane_bwd_fwd_w(cx<R> *tb, VecView<R> pin
   }
 }
 
+// Pass C fused with the next pass A, with the plane held in registers and
+// transposed through a buffer of NX/NH columns: k_plane_bwd_w's and
+// k_plane_fwd_w's chunked forms back to back, same arithmetic as
+// k_plane_bwd_fwd_w.  The buffer is 1/NH of a plane, so NH = 2 at complex64
+// 128^2 (67 KB) lets two workgroups share a CU and one's HBM phases run
+// under the other's FFTs; waves_per_eu holds the register budget to what two
+// resident workgroups leave (T / 128 waves per SIMD: 128 VGPRs at T = 512).
+template <typename R, int NX, int T, int NH>
+__global__ void __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T / 128)))
+k_plane_bwd_fwd_h(cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi, long N, int nz,
+                  const cx<R> *twg) {
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, P = PL::P;
+  static_assert(64 % G == 0 && NX % 8 == 0, "lines must not straddle wavefronts");
+  static_assert(NH > 1 && NX % NH == 0, "the chunked kernel splits the plane into NH chunks");
+  constexpr int W = NX / NH;
+  using L = PlaneLds<R, NX, NH>;
+  constexpr int LDc = L::LDc;
+  constexpr int RX = T / G;
+  constexpr int NR = (NX + RX - 1) / RX;
+  constexpr int NCR = (W + RX - 1) / RX;
+  constexpr int RS = NX + 1;
+  static_assert(RX * RS <= NX * LDc, "row exchange slices exceed the chunk buffer");
+  extern __shared__ char smem_raw[];
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
+  cx<R> *tw = buf + NX * LDc;
+  const long vec = blockIdx.x / nz;
+  const long z = blockIdx.x % nz;
+  const long pbase = z * (long)NX * NX;
+  const int tl = threadIdx.x / G, g0 = threadIdx.x % G;
+  const int g = g0;
+  constexpr int r0b = Pass<NX, 1>::radix(0);
+  constexpr int rlb = Pass<NX, 1>::radix(P - 1);
+  constexpr int rlf = Pass<NX, -1>::radix(P - 1);
+  TwTable<NX, 1, R> tcb;
+  TwTable<NX, -1, R> tcf;
+  cx<R> v[NR][E];  // the plane's rows, x in registers
+  // ---- inverse y FFT of the columns, chunk by chunk; each thread gathers
+  // the chunk's x positions of the rows it owns
+#pragma unroll
+  for (int c = 0; c < NH; c++) {
+    cx<R> w[NCR][E];
+#pragma unroll
+    for (int cr = 0; cr < NCR; cr++) {
+      const int cl = cr * RX + tl;
+      if (cl < W) {
+        const cx<R> *src = tb + vec * N + pbase + (long)(c * W + cl) * NX;
+#pragma unroll
+        for (int b = 0; b < E / r0b; b++)
+#pragma unroll
+          for (int k = 0; k < r0b; k++) w[cr][b * r0b + k] = src[in_idx<NX, 1, 0>(g, b, k)];
+      }
+    }
+    if (c == 0) load_tw<R, NX>(tw, twg);
+    __syncthreads();  // twiddles loaded / the previous chunk's gather is done
+    tcb.init(g, tw);
+#pragma unroll
+    for (int cr = 0; cr < NCR; cr++) {
+      const int cl = cr * RX + tl;
+      const bool act = cl < W;
+      const int cs = act ? cl : 0;
+      auto at = [&](int i) -> cx<R> & { return buf[L::rowbase(i) + cs]; };
+      run_passes_c<NX, 1, 0, P, WaveSync>(w[cr], g, act, tcb, at);
+      WaveSync::sync();
+      write_out<NX, 1, P - 1>(w[cr], g, act, at);
+    }
+    __syncthreads();  // the transpose
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int row = r * RX + tl;
+      if (row < NX) {
+#pragma unroll
+        for (int b = 0; b < E / r0b; b++)
+#pragma unroll
+          for (int k = 0; k < r0b; k++) {
+            const int x = in_idx<NX, 1, 0>(g, b, k);
+            if (x / W == c) v[r][b * r0b + k] = buf[L::rowbase(row) + (x - c * W)];
+          }
+      }
+    }
+  }
+  __syncthreads();  // the buffer is free for the row exchanges
+  // ---- rows: inverse x FFT, epilogue, store psi_j, forward x FFT of psi_j.
+  // Exchanges use slice tl of the buffer.  V and psi_{j-1} are loaded after
+  // the FFT (the other workgroup on the CU covers the latency); psi is in
+  // planar form (re and im arrays), and no partial sums are formed (the
+  // filter loop asks for none).
+  const R alpha = (R)epi.alpha, beta = (R)epi.beta;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int g = opaque_int(g0);  // per-round index math (see opaque_int)
+    tcb.init(g, tw);
+    tcf.init(g, tw);
+    const int row = r * RX + tl;
+    const bool act = row < NX;
+    const int rs = act ? row : 0;
+    auto at = [&](int i) -> cx<R> & { return buf[tl * RS + lds_swz(i)]; };
+    run_passes_c<NX, 1, 0, P, WaveSync>(v[r], g, act, tcb, at);
+    if (act) {
+      const long ro = pbase + (long)rs * NX;
+      const R *pr = pin.re + vec * pin.vstride + ro, *pi = pin.im + vec * pin.vstride + ro;
+      R *qr = out.re + vec * out.vstride + ro, *qi = out.im + vec * out.vstride + ro;
+      const R *vp = epi.pot + ro;
+      R V[E], pre[E], pim[E];
+#pragma unroll
+      for (int b = 0; b < E / rlb; b++)
+#pragma unroll
+        for (int k = 0; k < rlb; k++) {
+          const int x = out_idx<NX, 1, P - 1>(g, b, k);
+          V[b * rlb + k] = vp[x];
+          pre[b * rlb + k] = pr[x];
+          pim[b * rlb + k] = pi[x];
+        }
+#pragma unroll
+      for (int b = 0; b < E / rlb; b++)
+#pragma unroll
+        for (int k = 0; k < rlb; k++) {
+          const int e = b * rlb + k, x = out_idx<NX, 1, P - 1>(g, b, k);
+          cx<R> o;  // the arithmetic of epilogue()
+          o.re = alpha * (v[r][e].re + V[e] * pre[e]) - beta * pre[e];
+          o.im = alpha * (v[r][e].im + V[e] * pim[e]) - beta * pim[e];
+          qr[x] = o.re;
+          qi[x] = o.im;
+          v[r][e] = o;
+        }
+    }
+    // psi_j's row back through the slice into the forward input order
+    WaveSync::sync();
+    write_out<NX, 1, P - 1>(v[r], g, act, at);
+    WaveSync::sync();
+    read_in<NX, -1, 0>(v[r], g, act, at);
+    run_passes_c<NX, -1, 0, P, WaveSync>(v[r], g, act, tcf, at);
+  }
+  // ---- forward y FFT of the columns (pass A of step j+1), chunk by chunk
+#pragma unroll
+  for (int c = 0; c < NH; c++) {
+    __syncthreads();  // row exchanges / the previous chunk's columns are done
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int row = r * RX + tl;
+      if (row < NX) {
+#pragma unroll
+        for (int b = 0; b < E / rlf; b++)
+#pragma unroll
+          for (int k = 0; k < rlf; k++) {
+            const int kx = out_idx<NX, -1, P - 1>(g, b, k);
+            if (kx / W == c) buf[L::rowbase(row) + (kx - c * W)] = v[r][b * rlf + k];
+          }
+      }
+    }
+    __syncthreads();  // the transpose
+#pragma unroll
+    for (int cr = 0; cr < NCR; cr++) {
+      const int g = opaque_int(g0);
+      tcf.init(g, tw);
+      const int cl = cr * RX + tl;
+      const bool act = cl < W;
+      const int cs = act ? cl : 0;
+      cx<R> y[E];
+      auto at = [&](int i) -> cx<R> & { return buf[L::rowbase(i) + cs]; };
+      read_in<NX, -1, 0>(y, g, act, at);
+      run_passes_c<NX, -1, 0, P, WaveSync>(y, g, act, tcf, at);
+      if (act) {
+        cx<R> *dst = tb + vec * N + pbase + (long)(c * W + cs) * NX;
+#pragma unroll
+        for (int b = 0; b < E / rlf; b++)
+#pragma unroll
+          for (int k = 0; k < rlf; k++) dst[out_idx<NX, -1, P - 1>(g, b, k)] = y[b * rlf + k];
+      }
+    }
+  }
+}
+
 // ================================================================= dispatch
 template <int n>
 constexpr bool plane_colmajor() {
@@ -964,9 +1137,11 @@ struct HOps {
   // k_rows_bwd + the next step's k_rows_fwd in one kernel
   static bool rows_bwd_fwd(int nx, cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
                            long nrows, long N, int nvec, const cx<R> *tw, hipStream_t st);
-  // pass C fused with the next pass A; false where the plane needs chunks
+  // pass C fused with the next pass A; false where the plane needs chunks.
+  // chunked: the two-workgroups-per-CU form where plane_fused_h has one
   static bool plane_bwd_fwd(int nxy, cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
-                            long N, int nz, int nvec, const cx<R> *tw, hipStream_t st);
+                            long N, int nz, int nvec, const cx<R> *tw, hipStream_t st,
+                            bool chunked);
 };
 
 #define SEP_LINE_SIZES(X) X(16) X(24) X(32) X(36) X(48) X(64) X(80) X(96) X(128) X(160)
@@ -1136,10 +1311,35 @@ static void plane_bwd_launch(const cx<R> *tb, VecView<R> pin, VecView<R> out, Ep
   }
 }
 
+// Grids whose fused kernel also has the chunked two-workgroups-per-CU form
+// (k_plane_bwd_fwd_h): where the whole-plane buffer takes more than half the
+// LDS and half a plane fits the registers of 512 threads.
+template <typename R, int n>
+constexpr bool plane_fused_h() {
+  return sizeof(R) == 4 && n == 128;
+}
+
 template <typename R, int n>
 static bool plane_bwd_fwd_launch(cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi, long N,
-                                 int nz, int nvec, const cx<R> *tw, hipStream_t st) {
+                                 int nz, int nvec, const cx<R> *tw, hipStream_t st, bool chunked) {
   if constexpr (plane_colmajor<n>() && plane_chunks<R, n>() == 1) {
+    if constexpr (plane_fused_h<R, n>()) {
+      if (chunked && !epi.part && !pin.c && pin.im && !out.c && out.im) {
+        constexpr int T = 512, NH = 2;
+        constexpr size_t lds = sizeof(cx<R>) * ((size_t)n * PlaneLds<R, n, NH>::LDc + n);
+        static_assert(2 * lds <= 160 * 1024, "two chunk buffers must share the LDS of a CU");
+        static bool attr = false;
+        if (!attr) {
+          hipFuncSetAttribute((const void *)k_plane_bwd_fwd_h<R, n, T, NH>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+          attr = true;
+        }
+        if (nvec == 0) return true;  // attribute only (before graph capture)
+        hipLaunchKernelGGL((k_plane_bwd_fwd_h<R, n, T, NH>), dim3((unsigned)(nz * nvec)), dim3(T),
+                           lds, st, tb, pin, out, epi, N, nz, tw);
+        return true;
+      }
+    }
     constexpr int T = plane_threads<R, n>();
     constexpr size_t lds = plane_lds<R, n>();
     static_assert(lds <= 160 * 1024, "plane buffer exceeds the LDS of a gfx950 workgroup");
@@ -1157,14 +1357,35 @@ static bool plane_bwd_fwd_launch(cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<
   return false;
 }
 
+bool plane_fused_chunked_supported(int n, int prec) {
+  if (prec == SEP_PREC_FP64) {
+    switch (n) {
+#define C(x) \
+  case x:    \
+    return plane_fused_h<double, x>();
+      SEP_PLANE_SIZES_F64(C)
+#undef C
+    }
+    return false;
+  }
+  switch (n) {
+#define C(x) \
+  case x:    \
+    return plane_fused_h<float, x>();
+    SEP_PLANE_SIZES_F32(C)
+#undef C
+  }
+  return false;
+}
+
 template <>
 bool HOps<float>::plane_bwd_fwd(int nxy, cx<float> *tb, VecView<float> pin, VecView<float> out,
                                 Epi<float> epi, long N, int nz, int nvec, const cx<float> *tw,
-                                hipStream_t st) {
+                                hipStream_t st, bool chunked) {
   switch (nxy) {
 #define C(n) \
   case n:    \
-    return plane_bwd_fwd_launch<float, n>(tb, pin, out, epi, N, nz, nvec, tw, st);
+    return plane_bwd_fwd_launch<float, n>(tb, pin, out, epi, N, nz, nvec, tw, st, chunked);
     SEP_PLANE_SIZES_F32(C)
 #undef C
   }
@@ -1173,11 +1394,11 @@ bool HOps<float>::plane_bwd_fwd(int nxy, cx<float> *tb, VecView<float> pin, VecV
 template <>
 bool HOps<double>::plane_bwd_fwd(int nxy, cx<double> *tb, VecView<double> pin,
                                  VecView<double> out, Epi<double> epi, long N, int nz, int nvec,
-                                 const cx<double> *tw, hipStream_t st) {
+                                 const cx<double> *tw, hipStream_t st, bool chunked) {
   switch (nxy) {
 #define C(n) \
   case n:    \
-    return plane_bwd_fwd_launch<double, n>(tb, pin, out, epi, N, nz, nvec, tw, st);
+    return plane_bwd_fwd_launch<double, n>(tb, pin, out, epi, N, nz, nvec, tw, st, chunked);
     SEP_PLANE_SIZES_F64(C)
 #undef C
   }
@@ -1348,7 +1569,8 @@ bool hpsi_fused_supported(const GridDev &gd) {
   if (!split_plane(gd, prec)) return hop_supported(gd.nx) && hop_supported(gd.ny);
   VecView<R> none{nullptr, nullptr, nullptr, 0};
   Epi<R> epi{nullptr, 0.0, 0.0, nullptr};
-  return HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, nullptr, nullptr);
+  return HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, nullptr, nullptr,
+                                false);
 }
 
 template <typename R>
@@ -1394,7 +1616,8 @@ int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx
   Epi<R> epi{pot, alpha, beta, nullptr};
   bool ok;
   if (plane)
-    ok = fwd_next ? HOps<R>::plane_bwd_fwd(nx, tb, in, out, epi, N, nz, nvec, twx, st)
+    ok = fwd_next ? HOps<R>::plane_bwd_fwd(nx, tb, in, out, epi, N, nz, nvec, twx, st,
+                                           gd.fused_chunked[prec])
                   : HOps<R>::plane_bwd(nx, tb, in, out, epi, N, nz, nvec, twx, st, nullptr);
   else
     ok = fwd_next
@@ -1427,7 +1650,8 @@ int hpsi_prepare(const GridDev &gd) {
   Epi<R> epi{nullptr, 0.0, 0.0, nullptr};
   HOps<R>::plane_fwd(gd.nx, none, nullptr, gd.N, gd.nz, 0, tw, nullptr);
   HOps<R>::plane_bwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr, nullptr);
-  HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr);
+  HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr, false);
+  HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr, true);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 template int hpsi_prepare<double>(const GridDev &);

@@ -26,10 +26,14 @@ struct GridDev {
   // the filter loop runs pass C and the next pass A as separate kernels
   // (test hook SEP_SEPARATE_PASSES) instead of the fused plane kernel
   bool no_fuse;
+  // per precision: the fused plane kernel takes its chunked form (half a
+  // plane of LDS, two workgroups per CU) where one exists
+  bool fused_chunked[2];
 };
 
 bool hop_supported(int n);
 bool plane_supported(int nx, int ny, int prec);
+bool plane_fused_chunked_supported(int n, int prec);
 
 template <typename R>
 int hpsi_prepare(const GridDev &gd);
