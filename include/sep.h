@@ -345,6 +345,56 @@ int sep_ortho_fp32(sep_ctx *ctx, long m, double cutoff, double *S, long *kept);
  * enlarged basis holds fewer than m directions. */
 int sep_refine(sep_ctx *ctx, long m, double *eval, double *sige);
 
+/* ------------------------------------------------------ spin-orbit spinors */
+/* The reference's spin-orbit filter (spinors/filterSpinors), FP64.  A spinor
+ * is four real rows of N doubles: up.re, up.im, dn.re, dn.im (the
+ * reference's zomplex[2 ngrid], up then down, de-interleaved).  The kinetic
+ * clamp of that program is Ekinmax = 20 Ha (spinors/filterSpinors/init.c:23):
+ * open the context with ekinmax = 20 for its decks. */
+/* init_list (spinors/filterSpinors/init.c:114-183): the grid points within
+ * Rnlcut^2 = 0.49 * 6 ln 10 of each atom, with Vr = Vso exp(-r^2 / 0.49), r,
+ * r^2 and the l = 1 harmonics; sigma = dx (init.c:96-97).  vso[a] is the
+ * atom's spin-orbit strength (read.c:33-51; 0 for the passivating
+ * ligands): atoms with Vso = 0 contribute exactly nothing and get no list.
+ * nentries (optional) gets the number of (atom, point) entries. */
+int sep_so_setup(sep_ctx *ctx, long natom, const double *xyz, const double *vso,
+                 long *nentries);
+/* hamiltonian() (spinors/filterSpinors/hamiltonian.c:8-50, nonlocal_potential
+ * :79-192, the flagkb = 0 form): out = (T + V + V_so) in for nspinor device
+ * spinors (in != out). */
+int sep_apply_h_spinor_dev(sep_ctx *ctx, const void *in_dev, void *out_dev, long nspinor);
+/* get_energy_range() (spinors/filterSpinors/energy.c:53-112), fixed seed
+ * -874917403: Vmin = Emin - 0.2 |Emin|, dE = 1.2 Emax - Vmin; iters (optional,
+ * 2 values) gets the iterations of the Emin and Emax loops. */
+int sep_energy_range_spinor(sep_ctx *ctx, double *vmin, double *dE, long *iters);
+/* filtering() for every random spinor (spinors/filterSpinors/filter.c:5-83,
+ * main.c:84-107): nspinor device spinors, an (ms x nc, real: the Newton
+ * coefficients of the scalar filter, coeff.c is the same), zn (nc).  The
+ * ms * nspinor filtered spinors, each normalized, become the resident state
+ * block, spinor jns * ms + jms as the reference's psitot. */
+int sep_filter_spinor_dev(sep_ctx *ctx, const void *psi0_dev, long nspinor, const double *an,
+                          const double *zn, long nc, long ms, double vmin, double dE);
+/* energy_all() / calc_sigma_E() (spinors/filterSpinors/energy.c:28-49,
+ * 117-147) on the first m resident spinors. */
+int sep_energy_spinor(sep_ctx *ctx, long m, double *ene);
+int sep_sigma_spinor(sep_ctx *ctx, long m, double *sige);
+/* Smallest S_i / S_0 sep_ortho_spinor resolves: it takes singular values
+ * from the FP64 complex Gram matrix, whose eigenvalues are exact to about
+ * 1e-16 of the largest. */
+#define SEP_SPINOR_SVD_FLOOR 1.0e-7
+/* portho() (spinors/filterSpinors/ortho.c:5-45): orthonormal basis of the
+ * span of the m resident spinors, directions with S_i / S_0 below
+ * max(cutoff, SEP_SPINOR_SVD_FLOOR) dropped; S (optional, m values) gets the
+ * singular values, kept the basis size.  The reference's zgesvd resolves
+ * S_i / S_0 down to its 1e-10 cutoff; this floor drops the directions
+ * between 1e-10 and 1e-7 of the largest as well. */
+int sep_ortho_spinor(sep_ctx *ctx, long m, double cutoff, double *S, long *kept);
+/* Hmat() (spinors/filterSpinors/Hmat.c:5-72): the complex Hermitian
+ * projected H on the m resident orthonormal spinors, its eigenvalues
+ * ascending into eval, the resident spinors replaced by the Ritz spinors
+ * (normalized, main.c:146). */
+int sep_rayleigh_ritz_spinor(sep_ctx *ctx, long m, double *eval);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,14 @@ SIGNATURES = {
     "sep_set_timing": (_I, [_V, _I]),
     "sep_set_filter_lanes": (_I, [_V, _I]),
     "sep_set_hpsi_family": (_I, [_V, _I, _I]),
+    "sep_so_setup": (_I, [_V, _L, _dp, _dp, _lp]),
+    "sep_apply_h_spinor_dev": (_I, [_V, _V, _V, _L]),
+    "sep_energy_range_spinor": (_I, [_V, _dp, _dp, _lp]),
+    "sep_filter_spinor_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D]),
+    "sep_energy_spinor": (_I, [_V, _L, _dp]),
+    "sep_sigma_spinor": (_I, [_V, _L, _dp]),
+    "sep_ortho_spinor": (_I, [_V, _L, _D, _dp, _lp]),
+    "sep_rayleigh_ritz_spinor": (_I, [_V, _L, _dp]),
     "sep_get_hpsi_family": (_I, [_V, _I, _ip]),
     "sep_autotune_ex": (_I, [_V, _I, _L, _dp, _ip, _ip]),
     "sep_trim": (_I, [_V]),
@@ -416,6 +424,63 @@ class Context:
         self._chk(lib().sep_autotune_ex(self._h, precision, nvec, _p(ms), ctypes.byref(fam),
                                         ctypes.byref(lanes)), "sep_autotune_ex")
         return ms, fam.value, lanes.value
+
+    # ---- spin-orbit spinors (FP64; a spinor is 4 rows of N: up.re, up.im, dn.re, dn.im)
+    def so_setup(self, xyz, vso):
+        """Spin-orbit neighbourhood lists (sep_so_setup); returns the entry count."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        v = np.ascontiguousarray(vso, dtype=np.float64)
+        n = ctypes.c_long(0)
+        self._chk(lib().sep_so_setup(self._h, len(v), _p(x), _p(v), ctypes.byref(n)),
+                  "sep_so_setup")
+        return n.value
+
+    def apply_h_spinor_dev(self, in_ptr, out_ptr, nspinor):
+        self._chk(lib().sep_apply_h_spinor_dev(self._h, ctypes.c_void_p(in_ptr),
+                                               ctypes.c_void_p(out_ptr), nspinor),
+                  "sep_apply_h_spinor_dev")
+
+    def energy_range_spinor(self):
+        """(Vmin, dE, (Emin iterations, Emax iterations)) (sep_energy_range_spinor)."""
+        vmin, dE = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        its = np.zeros(2, dtype=np.int64)
+        self._chk(lib().sep_energy_range_spinor(self._h, ctypes.byref(vmin), ctypes.byref(dE),
+                                                its.ctypes.data_as(_lp)), "sep_energy_range_spinor")
+        return vmin.value, dE.value, (int(its[0]), int(its[1]))
+
+    def filter_spinor_dev(self, psi0_ptr, nspinor, an, zn, vmin, dE):
+        an = np.ascontiguousarray(an, dtype=np.float64)
+        zn = np.ascontiguousarray(zn, dtype=np.float64)
+        self._chk(lib().sep_filter_spinor_dev(self._h, ctypes.c_void_p(psi0_ptr), nspinor, _p(an),
+                                              _p(zn), zn.size, an.shape[0], vmin, dE),
+                  "sep_filter_spinor_dev")
+
+    def energy_spinor(self, m):
+        e = np.zeros(m)
+        self._chk(lib().sep_energy_spinor(self._h, m, _p(e)), "sep_energy_spinor")
+        return e
+
+    def sigma_spinor(self, m):
+        s = np.zeros(m)
+        self._chk(lib().sep_sigma_spinor(self._h, m, _p(s)), "sep_sigma_spinor")
+        return s
+
+    def ortho_spinor(self, m, cutoff=1.0e-10):
+        S = np.zeros(m)
+        k = ctypes.c_long(0)
+        self._chk(lib().sep_ortho_spinor(self._h, m, cutoff, _p(S), ctypes.byref(k)),
+                  "sep_ortho_spinor")
+        return k.value, S
+
+    def rayleigh_ritz_spinor(self, m):
+        e = np.zeros(m)
+        self._chk(lib().sep_rayleigh_ritz_spinor(self._h, m, _p(e)), "sep_rayleigh_ritz_spinor")
+        return e
+
+    def spinors_download(self, m):
+        """The first m resident spinors as complex (m, 2, N) (up, down)."""
+        x = self.states_download(4 * m).reshape(m, 2, 2, -1)
+        return x[:, :, 0, :] + 1j * x[:, :, 1, :]
 
     def set_timing(self, on):
         """Sample kernel durations inside the filter loop (see sep_set_timing)."""

@@ -62,6 +62,11 @@ struct sep_ctx {
   int lanes = 0;
   hipStream_t lst[SEP_MAX_LANES - 1] = {};
   hipEvent_t lev[SEP_MAX_LANES] = {};
+  // spin-orbit lists of the spinor H (sep_so_setup) and the per-entry
+  // contributions of one application
+  SoLists so;
+  double *so_contrib = nullptr;
+  size_t so_contrib_bytes = 0;
   std::string err;
 };
 
@@ -274,6 +279,9 @@ int sep_close(sep_ctx *c) {
     if (p) hipFree(p);
   for (hipEvent_t e : c->tev)
     if (e) hipEventDestroy(e);
+  for (void *p : {(void *)c->so.off, (void *)c->so.jxyz, (void *)c->so.geo, (void *)c->so.pidx,
+                  (void *)c->so.poff, (void *)c->so.pent, (void *)c->so_contrib})
+    if (p) hipFree(p);
   if (c->blas) rocblas_destroy_handle(c->blas);
   if (c->dws) hipFree(c->dws);
   if (c->zpot) hipFree(c->zpot);
@@ -661,6 +669,9 @@ struct FilterRun {
   R *ring = nullptr;
   bool sampled = false;
   bool fused = false;  // split steps: tb carries the spectrum of the newest vector
+  // spinors (4 real rows each: up.re, up.im, dn.re, dn.im): each step adds
+  // dE_1 V_so psi_(j-1), and the finish leaves normalization to the caller
+  bool so = false;
   // where this run's launches go: its stream, its slice of the spectrum
   // buffer and its filtered states (a lane owns a range of complex vectors)
   hipStream_t st = nullptr;
@@ -682,7 +693,7 @@ static int zero_dev(void *p, size_t bytes, hipStream_t st) {
 template <typename R>
 static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool psi0_on_device,
                         long ns, const double *an, const double *zn, long nc, long ms, double vmin,
-                        double dE) {
+                        double dE, bool so = false) {
   const long N = c->gd.N;
   if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
   f.c = c;
@@ -763,7 +774,8 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
   f.zn.assign(zn, zn + nc);
   f.j = 1;
   // pass A of psi_0; every later pass A runs fused with the previous pass C
-  f.fused = nc > 1 && !c->gd.no_fuse && hpsi_fused_supported<R>(c->gd);
+  f.so = so;
+  f.fused = !so && nc > 1 && !c->gd.no_fuse && hpsi_fused_supported<R>(c->gd);
   if (f.fused) {
     VecView<R> v0{f.ring, f.ring + ss, nullptr, 2 * ss};
     if (hpsi_fwd<R>(c->gd, v0, f.tb, (int)f.nvec, f.st) != SEP_OK)
@@ -811,6 +823,17 @@ static int filter_step(FilterRun<R> &f) {
       return fail(c, SEP_ERR_HIP, "filter: H.psi step launch failed");
   } else {
     RCHK(apply_views<R>(c, in, out, (int)f.nvec, f.dE_1, beta, nullptr, sample ? c->tev : nullptr));
+    if constexpr (sizeof(R) == 8) {
+      if (f.so && c->so.natom > 0) {
+        // spinor s, component q of slot k at ring + (4 s + q) ss + k slot
+        const long nsp = f.ns / 4;
+        RCHK(ensure(c, (void **)&c->so_contrib, &c->so_contrib_bytes,
+                    sizeof(double) * 4 * (size_t)c->so.nent * nsp));
+        if (so_apply_dev(c->so, f.ring + sp * slot, ss, 4 * ss, f.ring + sj * slot, ss, 4 * ss,
+                         nsp, f.dE_1, c->so_contrib, f.st) != SEP_OK)
+          return fail(c, SEP_ERR_HIP, "filter: spin-orbit launch failed");
+      }
+    }
   }
   f.sampled |= sample;
   if (sj == K - 1 || j == f.nc - 1) {
@@ -848,6 +871,7 @@ static int filter_finish(FilterRun<R> &f, int rc) {
   f.an_d = nullptr;
   f.cf_d = nullptr;
   if (rc != SEP_OK) return rc;
+  if (f.so) return SEP_OK;  // spinors: normalized whole by the caller
   return normalize_dev(c, c->states, f.ns * f.ms);  // filter.c:33
 }
 
@@ -879,9 +903,9 @@ static int filter_lanes(sep_ctx *c, const FilterRun<R> &f) {
 template <typename R>
 static int filter_impl(sep_ctx *c, const double *psi0, bool psi0_on_device, long ns,
                        const double *an, const double *zn, long nc, long ms, double vmin,
-                       double dE) {
+                       double dE, bool so = false) {
   FilterRun<R> f;
-  int rc = filter_start<R>(f, c, psi0, psi0_on_device, ns, an, zn, nc, ms, vmin, dE);
+  int rc = filter_start<R>(f, c, psi0, psi0_on_device, ns, an, zn, nc, ms, vmin, dE, so);
   const int L = rc == SEP_OK ? filter_lanes<R>(c, f) : 1;
   if (L <= 1) {
     while (rc == SEP_OK && f.j < nc) rc = filter_step<R>(f);
@@ -1915,3 +1939,381 @@ extern "C" int sep_rotate_dev(sep_ctx *c, int precision, const void *A, long m, 
   HIPCHK(c, hipStreamSynchronize(c->st));
   return SEP_OK;
 }
+
+// ================================================================ spinors
+// The reference's spin-orbit filter (spinors/filterSpinors): FP64 spinors
+// stored as four real rows of N (up.re, up.im, dn.re, dn.im), the kinetic
+// and local parts on the two complex components through the H.psi passes,
+// V_so through sep_spinor.hip.  The complex subspace stage runs on the real
+// matrix-core kernels: with X the spinors flattened to 4N reals and iX the
+// spinors times i, <x_s|y_t> = X_s.Y_t + i (iX_s).Y_t, and a complex
+// combination sum_s M_st x_s is the real combination of [X | iX] with the
+// stacked coefficients [Re M; Im M].
+
+static int so_contrib_buf(sep_ctx *c, long nsp) {
+  return ensure(c, (void **)&c->so_contrib, &c->so_contrib_bytes,
+                sizeof(double) * 4 * (size_t)(c->so.nent > 0 ? c->so.nent : 1) * nsp);
+}
+
+// dst = H src on m spinors (dst != src), 32 spinors (64 complex vectors) per pass
+static int apply_spinors(sep_ctx *c, const double *src, double *dst, long m) {
+  const long N = c->gd.N, n4 = 4 * N;
+  for (long s0 = 0; s0 < m; s0 += 32) {
+    const long nb = m - s0 < 32 ? m - s0 : 32;
+    double *x = const_cast<double *>(src) + s0 * n4, *y = dst + s0 * n4;
+    VecView<double> in{x, x + N, nullptr, 2 * N}, out{y, y + N, nullptr, 2 * N};
+    RCHK(apply_views<double>(c, in, out, (int)(2 * nb), 1.0, 0.0, nullptr));
+    if (c->so.natom > 0) {
+      RCHK(so_contrib_buf(c, nb));
+      if (so_apply_dev(c->so, x, N, n4, y, N, n4, nb, 1.0, c->so_contrib, c->st) != SEP_OK)
+        return fail(c, SEP_ERR_HIP, "spin-orbit launch failed");
+    }
+  }
+  return SEP_OK;
+}
+
+// out[i] = x_i . y_i over 4N reals (device out, m values)
+static int spinor_dots(sep_ctx *c, const double *x, const double *y, long m, double *out_d) {
+  const long n4 = 4 * c->gd.N, nb = (n4 + 16383) / 16384;
+  RCHK(ensure_scr(c, (size_t)nb * m + 4 * (size_t)m));
+  return dots_dev(x, n4, y, n4, m, n4, out_d, c->scr, nb * m, c->st) == SEP_OK
+             ? SEP_OK
+             : fail(c, SEP_ERR_HIP, "dot launch failed");
+}
+
+// normalize_all for spinors (spinors/filterSpinors/norm.c:40-62)
+static int normalize_spinors(sep_ctx *c, double *x, long m) {
+  if (m <= 0) return SEP_OK;
+  const long n4 = 4 * c->gd.N, nb = (n4 + 16383) / 16384;
+  RCHK(ensure_scr(c, (size_t)nb * m + 2 * (size_t)m));
+  double *out = c->scr + nb * m;
+  RCHK(spinor_dots(c, x, x, m, out));
+  std::vector<double> nrm(m);
+  HIPCHK(c, hipMemcpyAsync(nrm.data(), out, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (long i = 0; i < m; i++) nrm[i] = nrm[i] != 0.0 ? 1.0 / sqrt(nrm[i] * c->dv) : 0.0;
+  HIPCHK(c, hipMemcpyAsync(out + m, nrm.data(), sizeof(double) * m, hipMemcpyHostToDevice, c->st));
+  RCHK(scale_rows_dev(x, n4, m, n4, out + m, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return SEP_OK;
+}
+
+// Eigen-decomposition of the Hermitian n x n host matrix A (column-major
+// complex, A[r + c n]) on the device (rocSOLVER zheevd): eigenvalues
+// ascending into w, eigenvectors back into A (column j = vector j).
+static int dev_heev(sep_ctx *c, long n, std::vector<rocblas_double_complex> &A, double *w,
+                    const char *what) {
+  RCHK(dense_init(c, n));
+  RCHK(ensure_scr(c, 2 * (size_t)n * n));
+  rocblas_double_complex *A_d = reinterpret_cast<rocblas_double_complex *>(c->scr);
+  double *D = reinterpret_cast<double *>(c->dws), *E = D + n;
+  rocblas_int *info_d = reinterpret_cast<rocblas_int *>(E + n);
+  HIPCHK(c, hipMemcpyAsync(A_d, A.data(), sizeof(rocblas_double_complex) * n * n,
+                           hipMemcpyHostToDevice, c->st));
+  rocblas_status rs = rocsolver_zheevd(c->blas, rocblas_evect_original, rocblas_fill_upper,
+                                       (rocblas_int)n, A_d, (rocblas_int)n, D, E, info_d);
+  if (rs != rocblas_status_success)
+    return fail(c, SEP_ERR_HIP, "%s: rocsolver_zheevd status %d", what, (int)rs);
+  rocblas_int info = 0;
+  HIPCHK(c, hipMemcpyAsync(w, D, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(A.data(), A_d, sizeof(rocblas_double_complex) * n * n,
+                           hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(&info, info_d, sizeof(info), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (info != 0) return fail(c, SEP_ERR_NUMERIC, "%s: eigensolver did not converge (%d)", what, (int)info);
+  return SEP_OK;
+}
+
+// Complex m x m matrix A(s, t) = (Re + i Im)(s, t) * scale from two
+// row-major real device matrices, Hermitian part, column-major.
+static int complex_from_parts(sep_ctx *c, const double *re_d, const double *im_d, long m,
+                              double scale, std::vector<rocblas_double_complex> &A) {
+  std::vector<double> r((size_t)m * m), im((size_t)m * m);
+  HIPCHK(c, hipMemcpyAsync(r.data(), re_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(im.data(), im_d, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  A.assign((size_t)m * m, rocblas_double_complex(0.0, 0.0));
+  for (long s = 0; s < m; s++)
+    for (long t = 0; t < m; t++) {
+      const double ar = 0.5 * (r[s * m + t] + r[t * m + s]) * scale;
+      const double ai = 0.5 * (im[s * m + t] - im[t * m + s]) * scale;
+      A[s + t * m] = rocblas_double_complex(ar, ai);
+    }
+  return SEP_OK;
+}
+
+// Out (k spinors) = sum_s x_s Z(s, j) for the complex coefficients Z
+// (column-major m x k, scaled per column by sc[j]): one real combination of
+// the 2m rows [X | iX] at XX.
+static int combine_spinors(sep_ctx *c, const double *XX, long m,
+                           const std::vector<rocblas_double_complex> &Z, long ldz,
+                           const std::vector<long> &cols, const std::vector<double> &sc,
+                           double *Out) {
+  const long k = (long)cols.size(), n4 = 4 * c->gd.N;
+  std::vector<double> M((size_t)2 * m * k);
+  for (long s = 0; s < m; s++)
+    for (long j = 0; j < k; j++) {
+      const rocblas_double_complex z = Z[s + cols[j] * ldz];
+      M[s * k + j] = std::real(z) * sc[j];
+      M[(m + s) * k + j] = std::imag(z) * sc[j];
+    }
+  RCHK(ensure_scr(c, (size_t)2 * m * k));
+  HIPCHK(c, hipMemcpyAsync(c->scr, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice, c->st));
+  RCHK(sub_combine_k(c, XX, n4, 2 * m, c->scr, k, k, n4, Out, n4));
+  HIPCHK(c, hipStreamSynchronize(c->st));  // M outlives the copy
+  return SEP_OK;
+}
+
+// [X | iX] of the m resident spinors into c->states2 (2m spinors)
+static int stack_x_ix(sep_ctx *c, long m) {
+  const long n4 = 4 * c->gd.N;
+  HIPCHK(c, hipMemcpyAsync(c->states2, c->states, sizeof(double) * m * n4, hipMemcpyDeviceToDevice,
+                           c->st));
+  return spinor_times_i_dev(c->states, c->states2 + m * n4, c->gd.N, m, c->st) == SEP_OK
+             ? SEP_OK
+             : fail(c, SEP_ERR_HIP, "spinor times i launch failed");
+}
+
+extern "C" {
+
+int sep_so_setup(sep_ctx *c, long natom, const double *xyz, const double *vso, long *nentries) {
+  if (!c || natom < 0 || (natom > 0 && (!xyz || !vso))) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  SoHost h;
+  const double d[3] = {c->dx, c->dy, c->dz}, x0[3] = {c->xmin, c->ymin, c->zmin};
+  RCHK(so_build_host(c->gd.nx, c->gd.ny, c->gd.nz, d, x0, natom, xyz, vso, h));
+  for (void *p : {(void *)c->so.off, (void *)c->so.jxyz, (void *)c->so.geo, (void *)c->so.pidx,
+                  (void *)c->so.poff, (void *)c->so.pent})
+    if (p) hipFree(p);
+  c->so = SoLists();
+  SoLists &s = c->so;
+  // sigma = dx (spinors/filterSpinors/init.c:96-97), hamiltonian.c:98
+  const double sigma = c->dx;
+  s.c = 0.5 / sqrt(6.28318530717958647692 * sigma * sigma) * c->dv;
+  s.s1 = sqrt(0.5) / sigma;
+  s.nent = (long)h.jxyz.size();
+  s.npts = (long)h.pidx.size();
+  s.maxn = h.maxn;
+  if (nentries) *nentries = s.nent;
+  if (s.nent == 0) return SEP_OK;
+  s.natom = (int)(h.off.size() - 1);
+  auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+    HIPCHK(c, hipMalloc(dst, bytes));
+    HIPCHK(c, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return SEP_OK;
+  };
+  RCHK(up((void **)&s.off, h.off.data(), sizeof(long) * h.off.size()));
+  RCHK(up((void **)&s.jxyz, h.jxyz.data(), sizeof(int) * h.jxyz.size()));
+  RCHK(up((void **)&s.geo, h.geo.data(), sizeof(double) * h.geo.size()));
+  RCHK(up((void **)&s.pidx, h.pidx.data(), sizeof(int) * h.pidx.size()));
+  RCHK(up((void **)&s.poff, h.poff.data(), sizeof(long) * h.poff.size()));
+  RCHK(up((void **)&s.pent, h.pent.data(), sizeof(int) * h.pent.size()));
+  return SEP_OK;
+}
+
+int sep_apply_h_spinor_dev(sep_ctx *c, const void *in_dev, void *out_dev, long nspinor) {
+  if (!c || !in_dev || !out_dev || nspinor < 1 || in_dev == out_dev) return SEP_ERR_ARG;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  RCHK(apply_spinors(c, static_cast<const double *>(in_dev), static_cast<double *>(out_dev),
+                     nspinor));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return SEP_OK;
+}
+
+int sep_energy_range_spinor(sep_ctx *c, double *vmin, double *dE, long *iters) {
+  if (!c || !vmin || !dE) return SEP_ERR_ARG;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N, n4 = 4 * N;
+  sep_rng st;
+  sep_rng_reset(&st);
+  long idum = -874917403L;  // energy.c:58
+  std::vector<double> h((size_t)n4), comp((size_t)N);
+  auto init = [&]() {  // init_psi on the up, then the down component (energy.c:63-65)
+    std::fill(h.begin(), h.end(), 0.0);
+    for (int q = 0; q < 2; q++) {
+      sep_init_psi(&st, &idum, N, c->dv, comp.data());
+      std::copy(comp.begin(), comp.end(), h.begin() + 2 * q * N);
+    }
+  };
+  double *buf = nullptr;
+  HIPCHK(c, hipMalloc(&buf, sizeof(double) * 2 * n4));
+  double *x = buf, *hx = buf + n4;
+  int rc = SEP_OK;
+  double e[2] = {0.0, 0.0};
+  long its[2] = {0, 0};
+  const long nb = (n4 + 16383) / 16384;
+  for (int pass = 0; pass < 2 && rc == SEP_OK; pass++) {
+    init();
+    if (hipMemcpy(x, h.data(), sizeof(double) * n4, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = fail(c, SEP_ERR_HIP, "energy range: upload failed");
+      break;
+    }
+    if ((rc = apply_spinors(c, x, hx, 1)) != SEP_OK) break;
+    double ene = 0.1, old = 0.0;  // energy.c:68-69, 83-84
+    long i = 0;
+    for (; fabs((ene - old) / ene) > 1.0e-6 && i < 500; i++) {
+      // Emin: x <- x - tau H x; Emax: x <- H x; then normalized; H of the
+      // normalized iterate serves the energy and the next step
+      if (pass == 0) rc = axpby_dev(-0.025, hx, 1.0, x, n4, c->st);
+      else if (hipMemcpyAsync(x, hx, sizeof(double) * n4, hipMemcpyDeviceToDevice, c->st) != hipSuccess)
+        rc = SEP_ERR_HIP;
+      if (rc != SEP_OK) break;
+      if ((rc = normalize_spinors(c, x, 1)) != SEP_OK) break;
+      if ((rc = apply_spinors(c, x, hx, 1)) != SEP_OK) break;
+      if ((rc = ensure_scr(c, (size_t)nb + 4)) != SEP_OK) break;
+      if ((rc = spinor_dots(c, x, hx, 1, c->scr + nb)) != SEP_OK) break;
+      double dot = 0.0;
+      if (hipMemcpy(&dot, c->scr + nb, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = SEP_ERR_HIP;
+        break;
+      }
+      old = ene;
+      ene = dot * c->dv;
+    }
+    e[pass] = ene;
+    its[pass] = i;
+  }
+  hipFree(buf);
+  if (rc != SEP_OK) return rc;
+  const double emax = 1.2 * e[1], emin = e[0] - 0.2 * fabs(e[0]);  // energy.c:106-107
+  *vmin = emin;
+  *dE = emax - emin;
+  if (iters) {
+    iters[0] = its[0];
+    iters[1] = its[1];
+  }
+  return SEP_OK;
+}
+
+int sep_filter_spinor_dev(sep_ctx *c, const void *psi0_dev, long nspinor, const double *an,
+                          const double *zn, long nc, long ms, double vmin, double dE) {
+  if (!c || !psi0_dev || nspinor < 1 || !an || !zn || nc < 1 || ms < 1 || !(dE > 0.0))
+    return SEP_ERR_ARG;
+  if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  RCHK(filter_impl<double>(c, static_cast<const double *>(psi0_dev), true, 4 * nspinor, an, zn, nc,
+                           ms, vmin, dE, true));
+  const long N = c->gd.N, m = nspinor * ms;
+  RCHK(ensure_states(c, 4 * m, 2));
+  if (spinor_gather_rows_dev(c->states, c->states2, N, nspinor, ms, c->st) != SEP_OK)
+    return fail(c, SEP_ERR_HIP, "spinor reorder launch failed");
+  std::swap(c->states, c->states2);
+  std::swap(c->cap1, c->cap2);
+  return normalize_spinors(c, c->states, m);  // filter.c:29
+}
+
+int sep_energy_spinor(sep_ctx *c, long m, double *ene) {
+  if (!c || !ene || m < 1 || 4 * m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  RCHK(ensure_states(c, 4 * m, 2));
+  RCHK(apply_spinors(c, c->states, c->states2, m));
+  RCHK(ensure_scr(c, (size_t)m * ((4 * c->gd.N + 16383) / 16384) + 4 * (size_t)m));
+  std::vector<double> d(m);
+  double *out = c->scr + m * ((4 * c->gd.N + 16383) / 16384) + 2 * m;
+  RCHK(spinor_dots(c, c->states, c->states2, m, out));
+  HIPCHK(c, hipMemcpyAsync(d.data(), out, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (long i = 0; i < m; i++) ene[i] = d[i] * c->dv;  // energy.c:40-44
+  return SEP_OK;
+}
+
+int sep_sigma_spinor(sep_ctx *c, long m, double *sige) {
+  if (!c || !sige || m < 1 || 4 * m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  RCHK(ensure_states(c, 4 * m, 2));
+  RCHK(apply_spinors(c, c->states, c->states2, m));
+  const long nb = (4 * c->gd.N + 16383) / 16384;
+  RCHK(ensure_scr(c, (size_t)m * nb + 4 * (size_t)m));
+  std::vector<double> e(m), h2(m);
+  double *out = c->scr + m * nb + 2 * m;
+  RCHK(spinor_dots(c, c->states, c->states2, m, out));
+  HIPCHK(c, hipMemcpyAsync(e.data(), out, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  RCHK(spinor_dots(c, c->states2, c->states2, m, out));
+  HIPCHK(c, hipMemcpyAsync(h2.data(), out, sizeof(double) * m, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  // <psi|H^2|psi> = |H psi|^2 for Hermitian H (energy.c:117-147)
+  for (long i = 0; i < m; i++) {
+    const double ev = e[i] * c->dv;
+    sige[i] = sqrt(fabs(h2[i] * c->dv - ev * ev));
+  }
+  return SEP_OK;
+}
+
+int sep_ortho_spinor(sep_ctx *c, long m, double cutoff, double *S, long *kept) {
+  if (!c || !kept || m < 1 || 4 * m > c->cap1 || !(cutoff >= 0.0)) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long n4 = 4 * c->gd.N;
+  const double floor = cutoff > SEP_SPINOR_SVD_FLOOR ? cutoff : SEP_SPINOR_SVD_FLOOR;
+  long k = m;
+  for (int pass = 0; pass < 2; pass++) {
+    // pass 0: basis of the span (portho, ortho.c:28-40); pass 1 on that
+    // basis: the same map once more, to orthonormality at rounding level
+    RCHK(ensure_states(c, 8 * k, 2));
+    RCHK(stack_x_ix(c, k));
+    double *G_d = nullptr;
+    HIPCHK(c, hipMalloc(&G_d, sizeof(double) * 2 * k * k));
+    int rc = sub_gram_k(c, c->states, n4, k, c->states, n4, k, n4, false, G_d);
+    if (rc == SEP_OK)
+      rc = sub_gram_k(c, c->states2 + k * n4, n4, k, c->states, n4, k, n4, false, G_d + k * k);
+    std::vector<rocblas_double_complex> A;
+    if (rc == SEP_OK) rc = complex_from_parts(c, G_d, G_d + k * k, k, 1.0, A);
+    hipFree(G_d);
+    RCHK(rc);
+    std::vector<double> w(k);
+    RCHK(dev_heev(c, k, A, w.data(), "spinor ortho"));
+    std::vector<double> s(k);
+    for (long i = 0; i < k; i++) s[i] = sqrt(w[k - 1 - i] > 0.0 ? w[k - 1 - i] : 0.0);
+    if (!(s[0] > 0.0)) return fail(c, SEP_ERR_NUMERIC, "spinor ortho: all spinors are zero");
+    long kk = k;
+    if (pass == 0) {
+      for (long i = 0; i < k; i++)
+        if (s[i] / s[0] < floor) {  // ortho.c:34-39
+          kk = i;
+          break;
+        }
+      if (S) memcpy(S, s.data(), sizeof(double) * k);
+    }
+    std::vector<long> cols(kk);
+    std::vector<double> sc(kk);
+    for (long j = 0; j < kk; j++) {
+      cols[j] = k - 1 - j;
+      sc[j] = 1.0 / s[j];
+    }
+    RCHK(combine_spinors(c, c->states2, k, A, k, cols, sc, c->states));
+    k = kk;
+  }
+  *kept = k;
+  return normalize_spinors(c, c->states, k);  // main.c:137
+}
+
+int sep_rayleigh_ritz_spinor(sep_ctx *c, long m, double *eval) {
+  if (!c || !eval || m < 1 || 4 * m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long n4 = 4 * c->gd.N;
+  // states2: [X | iX | H X]
+  RCHK(ensure_states(c, 12 * m, 2));
+  RCHK(stack_x_ix(c, m));
+  double *HX = c->states2 + 2 * m * n4;
+  RCHK(apply_spinors(c, c->states, HX, m));
+  double *P_d = nullptr;
+  HIPCHK(c, hipMalloc(&P_d, sizeof(double) * 2 * m * m));
+  // P(s, t) = <x_s|H x_t> dv (Hmat.c:30-40)
+  int rc = sub_gram_k(c, c->states, n4, m, HX, n4, m, n4, false, P_d);
+  if (rc == SEP_OK) rc = sub_gram_k(c, c->states2 + m * n4, n4, m, HX, n4, m, n4, false, P_d + m * m);
+  std::vector<rocblas_double_complex> A;
+  if (rc == SEP_OK) rc = complex_from_parts(c, P_d, P_d + m * m, m, c->dv, A);
+  hipFree(P_d);
+  RCHK(rc);
+  RCHK(dev_heev(c, m, A, eval, "spinor Rayleigh-Ritz"));  // Hmat.c:45
+  std::vector<long> cols(m);
+  std::vector<double> sc(m, 1.0);
+  for (long j = 0; j < m; j++) cols[j] = j;
+  RCHK(combine_spinors(c, c->states2, m, A, m, cols, sc, c->states));  // Hmat.c:49-66
+  return normalize_spinors(c, c->states, m);  // main.c:146
+}
+
+}  // extern "C"

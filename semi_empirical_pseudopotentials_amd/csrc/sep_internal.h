@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/sep.h"
 #include "fft_core.h"
 #include "sep_host.h"
@@ -30,6 +32,39 @@ struct GridDev {
   // plane of LDS, two workgroups per CU) where one exists
   bool fused_chunked[2];
 };
+
+// Spin-orbit neighbourhood lists on the device (sep_so_setup,
+// spinors/filterSpinors/init.c:114-183), atoms with Vso != 0.
+struct SoLists {
+  int natom = 0;          // atoms with a list
+  long nent = 0;          // entries over all atoms
+  long *off = nullptr;    // [natom + 1] first entry of each atom
+  int *jxyz = nullptr;    // [nent] grid index of each entry
+  double *geo = nullptr;  // [nent][9] r, r2, Vr, y10 (re, im), y11 (re, im), y1-1 (re, im)
+  long npts = 0;          // grid points in some list
+  int *pidx = nullptr;    // [npts] their grid indices
+  long *poff = nullptr;   // [npts + 1] into pent
+  int *pent = nullptr;    // [nent] entries covering each point, atoms in order
+  double c = 0.0;         // dv / (2 sqrt(2 pi sigma^2)), hamiltonian.c:98
+  double s1 = 0.0;        // sigma_1 = sqrt(1/2) / sigma, init.c:97
+  long maxn = 0;          // longest list
+};
+struct SoHost {
+  std::vector<long> off, poff;
+  std::vector<int> jxyz, pidx, pent;
+  std::vector<double> geo;
+  long maxn = 0;
+};
+int so_build_host(int nx, int ny, int nz, const double d[3], const double x0[3], long natom,
+                  const double *xyz, const double *vso, SoHost &h);
+// out_s += alpha V_so in_s for nsp spinors: component c of spinor s at
+// in + s ss + c cs (out likewise); contrib holds nent * nsp * 4 doubles
+int so_apply_dev(const SoLists &so, const double *in, long cs, long ss, double *out, long ocs,
+                 long oss, long nsp, double alpha, double *contrib, hipStream_t st);
+int spinor_times_i_dev(const double *x, double *out, long N, long m, hipStream_t st);
+int spinor_gather_rows_dev(const double *src, double *dst, long N, long nsp, long ms,
+                           hipStream_t st);
+int axpby_dev(double a, const double *x, double b, double *y, long n, hipStream_t st);
 
 bool hop_supported(int n);
 bool plane_supported(int nx, int ny, int prec);

@@ -1,0 +1,130 @@
+"""Spin-orbit spinor filter (spinors/filterSpinors) on the GPU against the
+spinor oracle (oracle/spinor.py), FP64: the spinor H with the matrix-form
+nonlocal spin-orbit term to 1e-12, then the pipeline -- energy range,
+filter, ortho, Rayleigh-Ritz, sigma -- on a 16^3 grid with an In/As
+cluster (8 random spinors x 4 targets, 512 Newton steps).  The filtered spinors' energies (eval-filt) agree to 1e-10 Ha and
+the converged Ritz values (sigma < 1e-6) to 1e-9 Ha."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# spin-orbit strengths of In and As (spinors/filterSpinors/read.c:36, 51)
+VSO_IN = 0.48 * (6.057 / 0.529177) ** 3 / 8.0 / 27.2114
+VSO_AS = 3.15164354
+
+
+def _cluster():
+    a = 6.057 / 0.529177 / 4.0  # quarter of the InAs lattice constant, bohr
+    xyz = np.array([[0.0, 0.0, 0.0], [a, a, a], [-a, -a, a], [a, -a, -a], [-a, a, -a]])
+    xyz += np.array([0.13, -0.07, 0.05])
+    vso = np.array([VSO_IN, VSO_AS, VSO_AS, VSO_AS, VSO_AS])
+    return xyz, vso
+
+
+def _setup(sep, oracle, n, d, ekin=20.0):
+    from oracle import spinor as S
+    grid = (n, n, n)
+    origin = (-n * d / 2,) * 3
+    xyz, vso = _cluster()
+    lists = S.so_lists(grid, origin, (d, d, d), xyz, vso)
+    ksq = oracle.ksqr(n, n, n, d, d, d, ekin)
+    rng = np.random.default_rng(7)
+    X, Y, Z = np.meshgrid(*(np.arange(n) * d + origin[0],) * 3, indexing="ij")
+    V = (-0.6 * np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 8.0)).transpose(2, 1, 0).ravel()
+    V = V + 0.02 * rng.standard_normal(V.size)
+    ctx = sep.Context(n, n, n, d, d, d, *origin, ekinmax=ekin)
+    ctx.set_potential(V)
+    nent = ctx.so_setup(xyz, vso)
+    assert nent == sum(e["jxyz"].size for e in lists)
+    return S, grid, lists, ksq, V, ctx
+
+
+def _to_rows(psi):
+    """(m, 2, N) complex spinors -> (m, 4N) rows up.re, up.im, dn.re, dn.im."""
+    m = psi.shape[0]
+    out = np.empty((m, 4, psi.shape[2]))
+    out[:, 0], out[:, 1] = psi[:, 0].real, psi[:, 0].imag
+    out[:, 2], out[:, 3] = psi[:, 1].real, psi[:, 1].imag
+    return out.reshape(m, -1)
+
+
+def test_spinor_h_vs_oracle(sep, oracle):
+    import torch
+    n, d = 24, 0.5
+    print("case spinor H 24^3", flush=True)
+    S, grid, lists, ksq, V, ctx = _setup(sep, oracle, n, d)
+    with ctx:
+        rng = np.random.default_rng(11)
+        N = n ** 3
+        psi = rng.standard_normal((3, 2, N)) + 1j * rng.standard_normal((3, 2, N))
+        x = torch.from_numpy(_to_rows(psi)).to("cuda:0")
+        y = torch.empty_like(x)
+        ctx.apply_h_spinor_dev(x.data_ptr(), y.data_ptr(), 3)
+        got = y.cpu().numpy().reshape(3, 2, 2, N)
+        got = got[:, :, 0] + 1j * got[:, :, 1]
+    ref = np.array([S.hamiltonian(grid, ksq, V, lists, p, d ** 3, d) for p in psi])
+    so = np.array([S.so_apply(lists, p, d ** 3, d) for p in psi])
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("spinor H rel err", err, "|V_so psi| / |H psi|", np.abs(so).max() / np.abs(ref).max(),
+          flush=True)
+    assert np.abs(so).max() > 1e-3 * np.abs(ref).max()  # the spin-orbit term is not negligible
+    assert err < 1e-12
+
+
+def test_spinor_pipeline_vs_oracle(sep, oracle):
+    import torch
+    n, d = 16, 0.6
+    N = n ** 3
+    dv = d ** 3
+    print("case spinor pipeline 16^3", flush=True)
+    # a kinetic clamp of 6 Ha (the reference's 20 in the H test above) keeps
+    # dE small, so that 512 Newton steps resolve the lowest levels
+    S, grid, lists, ksq, V, ctx = _setup(sep, oracle, n, d, ekin=6.0)
+    with ctx:
+        vmin, dE, its = ctx.energy_range_spinor()
+        ovmin, odE, oits = S.energy_range(grid, ksq, V, lists, dv, d)
+        print("energy range gpu", vmin, dE, its, "oracle", ovmin, odE, oits, flush=True)
+        assert its == oits
+        assert abs(vmin - ovmin) < 1e-9 * abs(ovmin) and abs(dE - odE) < 1e-9 * odE
+        nc, ms, ns = 512, 4, 8
+        zn, _ = sep.leja_points(nc)
+        # targets over the low end of the spectrum, from Emin (Vmin = Emin - 0.2 |Emin|)
+        e0 = ovmin / 1.2 if ovmin < 0 else ovmin / 0.8
+        el = np.linspace(e0, e0 + 0.4, ms)
+        an = sep.newton_coeffs(zn, el, odE, ovmin, (nc / (2.5 * odE)) ** 2)
+        rng = oracle.Rng()
+        idum = np.array([-1001], dtype=np.int64)
+        psi0 = np.array([S.init_spinor(rng, idum, N, dv) for _ in range(ns)])
+        x0 = torch.from_numpy(_to_rows(psi0)).to("cuda:0")
+        ctx.filter_spinor_dev(x0.data_ptr(), ns, an, zn, ovmin, odE)
+        m = ns * ms
+        e_filt = ctx.energy_spinor(m)
+        filt = ctx.spinors_download(m)
+        kept, sv = ctx.ortho_spinor(m)
+        ev = ctx.rayleigh_ritz_spinor(kept)
+        sg = ctx.sigma_spinor(kept)
+    print("case spinor pipeline oracle", flush=True)
+    ofilt = np.concatenate([S.filter_spinor(grid, ksq, V, lists, dv, d, an, zn, odE, ovmin, p)
+                            for p in psi0])
+    ref_e = np.array([float(np.sum(np.conj(p) * S.hamiltonian(grid, ksq, V, lists, p, dv, d)).real
+                            * dv) for p in ofilt])
+    ferr = np.abs(filt - ofilt).max() / np.abs(ofilt).max()
+    print("filtered spinors rel err", ferr, "eval-filt max |dE|", np.abs(e_filt - ref_e).max(),
+          flush=True)
+    assert ferr < 1e-9
+    assert np.abs(e_filt - ref_e).max() < 1e-10
+    U, osv, ok = S.portho(ofilt)
+    U = np.array([u / np.sqrt(S.spinor_norm2(u, dv)) for u in U])
+    oev, opsi = S.hmat(grid, ksq, V, lists, dv, d, U)
+    osg = S.sigma_e(grid, ksq, V, lists, dv, d, opsi)
+    print("kept gpu", kept, "oracle", ok, "S ratio gpu", sv[:kept][-1] / sv[0], flush=True)
+    np.testing.assert_allclose(sv[:kept], osv[:kept], rtol=1e-6, atol=1e-7 * osv[0])
+    conv = osg < 1e-6
+    assert conv.sum() >= 4
+    err = np.array([np.abs(ev - e).min() for e in oev[conv]])
+    print("converged", conv.sum(), "max |dE| Ha", err.max(), flush=True)
+    assert err.max() < 1e-9
+    # sigma of the GPU Ritz spinors agrees where converged
+    g = np.array([sg[np.abs(ev - e).argmin()] for e in oev[conv]])
+    assert np.all(g < 2e-6)
