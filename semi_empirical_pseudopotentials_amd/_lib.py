@@ -103,6 +103,7 @@ SIGNATURES = {
     "sep_set_filter_lanes": (_I, [_V, _I]),
     "sep_set_hpsi_family": (_I, [_V, _I, _I]),
     "sep_so_setup": (_I, [_V, _L, _dp, _dp, _lp]),
+    "sep_refine_window": (_I, [_V, _L, _dp, _dp, _D, _D, _I, _lp]),
     "sep_apply_h_spinor_dev": (_I, [_V, _V, _V, _L]),
     "sep_energy_range_spinor": (_I, [_V, _dp, _dp, _lp]),
     "sep_filter_spinor_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D]),
@@ -585,6 +586,17 @@ class Context:
                                   _p(dR), _p(ds), _p(dtab))
         self._chk(rc, "sep_hf_adjoint")
         return dR, ds, dtab
+
+    def refine_window(self, ev, sg, lo, hi, passes=2):
+        """sep_refine_window: more correction passes on the resident Ritz pairs
+        with energies in [lo, hi]; returns their (ev, sigma), now the first
+        len(ev) resident states."""
+        ev = np.array(ev, dtype=np.float64)
+        sg = np.array(sg, dtype=np.float64)
+        kw = ctypes.c_long(0)
+        self._chk(lib().sep_refine_window(self._h, ev.size, _p(ev), _p(sg), lo, hi, passes,
+                                          ctypes.byref(kw)), "sep_refine_window")
+        return ev[:kw.value].copy(), sg[:kw.value].copy()
 
     def ortho_fp32(self, m, cutoff=1.0e-10):
         """ortho of the FP32 pipeline (sep_ortho_fp32): FP64 Gram eigenbasis with the

@@ -118,6 +118,22 @@ static int ensure_states(sep_ctx *c, long m, int which) {
   (which == 1 ? c->cap1 : c->cap2) = m;
   return SEP_OK;
 }
+// ensure_states that keeps the first `keep` rows of the buffer
+static int grow_states(sep_ctx *c, long m, int which, long keep) {
+  long &cap = which == 1 ? c->cap1 : c->cap2;
+  double *&p = which == 1 ? c->states : c->states2;
+  if (p && cap >= m) return SEP_OK;
+  double *q = nullptr;
+  HIPCHK(c, hipMalloc(&q, sizeof(double) * (size_t)m * c->gd.N));
+  if (p && keep > 0)
+    HIPCHK(c, hipMemcpyAsync(q, p, sizeof(double) * (size_t)(keep < cap ? keep : cap) * c->gd.N,
+                             hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (p) hipFree(p);
+  p = q;
+  cap = m;
+  return SEP_OK;
+}
 static int ensure_scr(sep_ctx *c, size_t len) {
   size_t cap = c->scr_len * sizeof(double);
   RCHK(ensure(c, (void **)&c->scr, &cap, len * sizeof(double)));
@@ -1352,6 +1368,51 @@ static int rr_polished(sep_ctx *c, long k, double *eval) {
   return normalize_dev(c, c->states, k);
 }
 
+// rr_polished that also leaves H applied to the Ritz vectors: the basis'
+// H Y rotated with the same coefficients (no further H application), both
+// blocks scaled to the normalized Ritz vectors.  After it c->states holds
+// the k Ritz vectors and c->states2 (rows 0..k) their H psi; sige gets
+// sqrt(|H psi|^2 dv - e^2).
+static int rr_polished_h(sep_ctx *c, long k, double *eval, double *sige) {
+  const long N = c->gd.N;
+  RCHK(ensure_states(c, 2 * k, 2));
+  RCHK(apply_states(c, c->states, c->states2, k, nullptr, nullptr));  // H y_i
+  RCHK(ensure_scr(c, 3 * k * k));
+  double *U_d = c->scr, *P_d = U_d + k * k, *W_d = P_d + k * k;
+  RCHK(sub_gram_sym(c, c->states, c->states, k, 1.0, U_d));
+  RCHK(sub_gram_sym(c, c->states, c->states2, k, 1.0, P_d));
+  RCHK(dev_chol_inv(c, k, U_d, "Rayleigh-Ritz: Gram factor"));
+  RCHK(sub_combine_k(c, U_d, k, k, P_d, k, k, k, W_d, k));
+  RCHK(sub_combine_k(c, W_d, k, k, U_d, k, k, k, P_d, k));
+  RCHK(symmetrize_dev(P_d, k, 1.0, c->st));
+  RCHK(dev_symeig(c, k, P_d, eval, "Rayleigh-Ritz"));
+  RCHK(sub_gram_k(c, U_d, k, k, P_d, k, k, k, false, W_d));
+  // Ritz vectors into states2 rows k..2k, their H psi over the basis in states
+  RCHK(sub_combine(c, c->states, k, W_d, k, k, c->states2 + k * N));
+  RCHK(sub_combine(c, c->states2, k, W_d, k, k, c->states));
+  HIPCHK(c, hipMemcpyAsync(c->states2, c->states2 + k * N, sizeof(double) * k * N,
+                           hipMemcpyDeviceToDevice, c->st));
+  std::swap(c->states, c->states2);  // states: Ritz vectors, states2: H psi
+  std::swap(c->cap1, c->cap2);
+  // normalization (filter/main.c:122) applied to both blocks, then sigma
+  const long nb = (N + 16383) / 16384;
+  RCHK(ensure_scr(c, (size_t)nb * k + 3 * (size_t)k));
+  double *out = c->scr + nb * k;
+  RCHK(dots_dev(c->states, N, c->states, N, k, N, out, c->scr, nb * k, c->st));
+  std::vector<double> nrm(k), h2(k);
+  HIPCHK(c, hipMemcpyAsync(nrm.data(), out, sizeof(double) * k, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (long i = 0; i < k; i++) nrm[i] = nrm[i] != 0.0 ? 1.0 / sqrt(nrm[i] * c->dv) : 0.0;
+  HIPCHK(c, hipMemcpyAsync(out + k, nrm.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->st));
+  RCHK(scale_rows_dev(c->states, N, k, N, out + k, c->st));
+  RCHK(scale_rows_dev(c->states2, N, k, N, out + k, c->st));
+  RCHK(dots_dev(c->states2, N, c->states2, N, k, N, out + 2 * k, c->scr, nb * k, c->st));
+  HIPCHK(c, hipMemcpyAsync(h2.data(), out + 2 * k, sizeof(double) * k, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (long i = 0; i < k; i++) sige[i] = sqrt(fabs(h2[i] * c->dv - eval[i] * eval[i]));
+  return SEP_OK;
+}
+
 extern "C" int sep_ortho_fp32(sep_ctx *c, long m, double cutoff, double *S, long *kept) {
   if (!c || m < 1 || m > c->cap1 || !kept || !(cutoff >= 0.0)) return SEP_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
@@ -1398,36 +1459,55 @@ static int precond_states(sep_ctx *c, const double *src, double *dst, long m, do
 // precond_states, shift SEP_REFINE_SHIFT), Rayleigh-Ritz in span{Psi, T}
 // (orthonormalised from the FP64 Gram with singular-value floor 1e-7), and
 // the m Ritz pairs of smallest energy variance kept, ascending in energy.
-extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
-  if (!c || !eval || !sige || m < 1 || m > c->cap1) return SEP_ERR_ARG;
-  HIPCHK(c, hipSetDevice(c->device));
+// One correction pass on the m resident Ritz pairs (eval ascending): a
+// Rayleigh-Ritz on [Psi, P R] (2m states), or [Psi, P R, Psi_prev] (3m: the
+// previous iterate of the same pairs, the momentum term of LOBPCG) when
+// prev is given, of whose pairs m are kept: with lo > hi those of smallest
+// variance, else those inside [lo, hi] first (smallest variance among
+// them), then the rest by variance.  Kept pairs ascending in energy at the
+// front of the state block; returns their count.
+static int refine_pass(sep_ctx *c, long m, double *eval, double *sige, double lo, double hi,
+                       long *kept, const double *prev = nullptr, bool have_h = false) {
   const long N = c->gd.N;
-  // [Psi; H Psi] in states2 (2m rows), then r_i = H psi_i - e_i psi_i
-  RCHK(ensure_states(c, 2 * m, 2));
+  const long nb = prev ? 3 : 2;  // blocks of the Rayleigh-Ritz basis
+  // [Psi; H Psi] in states2 (2m rows), then r_i = H psi_i - e_i psi_i.
+  // have_h: H Psi is already in states2 rows 0..m (the previous pass)
+  RCHK(ensure_states(c, nb * m, 2));
+  if (have_h)
+    HIPCHK(c, hipMemcpyAsync(c->states2 + m * N, c->states2, sizeof(double) * m * N,
+                             hipMemcpyDeviceToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->states2, c->states, sizeof(double) * m * N, hipMemcpyDeviceToDevice,
                            c->st));
-  RCHK(apply_states(c, c->states, c->states2 + m * N, m, nullptr, nullptr));
+  if (!have_h) RCHK(apply_states(c, c->states, c->states2 + m * N, m, nullptr, nullptr));
   RCHK(ensure_scr(c, m));
   HIPCHK(c, hipMemcpyAsync(c->scr, eval, sizeof(double) * m, hipMemcpyHostToDevice, c->st));
   RCHK(sub_scaled_rows_dev(c->states2 + m * N, N, c->states, N, m, N, c->scr, c->st));
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
   // t_i = P r_i into the second block of states2, copied back over r_i
-  // (states2 is the former states buffer: grow it to 2m rows first)
-  RCHK(ensure_states(c, 2 * m, 2));
+  // (states2 is the former states buffer: grow it to nb m rows first)
+  RCHK(ensure_states(c, nb * m, 2));
   RCHK(precond_states(c, c->states + m * N, c->states2 + m * N, m, SEP_REFINE_SHIFT));
   HIPCHK(c, hipMemcpyAsync(c->states + m * N, c->states2 + m * N, sizeof(double) * m * N,
                            hipMemcpyDeviceToDevice, c->st));
   RCHK(normalize_dev(c, c->states + m * N, m));
+  if (prev)
+    HIPCHK(c, hipMemcpyAsync(c->states + 2 * m * N, prev, sizeof(double) * m * N,
+                             hipMemcpyDeviceToDevice, c->st));
   long k = 0;
-  RCHK(eig_basis(c, 2 * m, SEP_FP32_SVD_FLOOR, nullptr, &k, false));
+  RCHK(eig_basis(c, nb * m, SEP_FP32_SVD_FLOOR, nullptr, &k, false));
   std::vector<double> ev(k), sg(k);
-  RCHK(rr_polished(c, k, ev.data()));
-  RCHK(sep_sigma(c, k, sg.data()));
-  // keep the m pairs of smallest variance, ascending in energy
+  RCHK(rr_polished_h(c, k, ev.data(), sg.data()));  // H psi of the pairs in states2
   std::vector<long> idx(k);
   for (long i = 0; i < k; i++) idx[i] = i;
-  std::stable_sort(idx.begin(), idx.end(), [&](long a, long b) { return sg[a] < sg[b]; });
+  const bool win = lo <= hi;
+  std::stable_sort(idx.begin(), idx.end(), [&](long a, long b) {
+    if (win) {
+      const bool ia = ev[a] >= lo && ev[a] <= hi, ib = ev[b] >= lo && ev[b] <= hi;
+      if (ia != ib) return ia;
+    }
+    return sg[a] < sg[b];
+  });
   const long keep = k < m ? k : m;
   idx.resize(keep);
   std::sort(idx.begin(), idx.end());  // Ritz values are ascending in index
@@ -1436,13 +1516,201 @@ extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
   for (long j = 0; j < keep; j++) {
     eval[j] = ev[idx[j]];
     sige[j] = sg[idx[j]];
+    if (idx[j] != j) {
+      HIPCHK(c, hipMemcpyAsync(c->states + j * N, c->states + idx[j] * N, sizeof(double) * N,
+                               hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(c->states2 + j * N, c->states2 + idx[j] * N, sizeof(double) * N,
+                               hipMemcpyDeviceToDevice, c->st));
+    }
+  }
+  for (long j = keep; j < m; j++) eval[j] = sige[j] = 0.0;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  *kept = keep;
+  return SEP_OK;  // states2 rows 0..keep: H psi of the kept pairs
+}
+
+extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
+  if (!c || !eval || !sige || m < 1 || m > c->cap1) return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  long keep = 0;
+  RCHK(refine_pass(c, m, eval, sige, 1.0, 0.0, &keep));
+  return keep == m ? SEP_OK : fail(c, SEP_ERR_NUMERIC, "refine: basis of %ld < %ld states", keep, m);
+}
+
+// One correction pass of sep_refine_window on the k pairs at the front of
+// the state block (X in states rows 0..k, H X in states2 rows 0..k, energies
+// ev): Rayleigh-Ritz on Z = [X, T, X_prev] (T = P R, R = H X - e X, X_prev
+// and H X_prev of the previous pass when prev is given) in coefficient
+// space -- Gram matrices Z Z^T, Z (H Z)^T, (H Z)(H Z)^T (matrix cores), G =
+// W L W^T reduced to the directions with l > floor^2 l_max, the reduced H
+// diagonalised, sigma^2 = c^T (HZ HZ^T) c - e^2 -- and only the k kept pairs
+// (inside [lo, hi] first, then by variance) rotated, with their H psi, both
+// normalized as filter/main.c:122.  One H application per pass (on T).
+// On return prev / hprev hold this pass's X and H X.
+static int window_pass(sep_ctx *c, long k, double *ev, double *sg, double lo, double hi,
+                       double *prev, double *hprev, bool have_prev) {
+  const long N = c->gd.N;
+  const long nz = have_prev ? 3 * k : 2 * k;
+  double *Z = c->states, *HZ = c->states2;
+  // T = P (H X - e X), normalized, into Z rows k..2k; H T into HZ rows k..2k
+  RCHK(ensure_scr(c, k));
+  HIPCHK(c, hipMemcpyAsync(Z + k * N, HZ, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->scr, ev, sizeof(double) * k, hipMemcpyHostToDevice, c->st));
+  RCHK(sub_scaled_rows_dev(Z + k * N, N, Z, N, k, N, c->scr, c->st));
+  RCHK(precond_states(c, Z + k * N, Z + 3 * k * N, k, SEP_REFINE_SHIFT));
+  HIPCHK(c, hipMemcpyAsync(Z + k * N, Z + 3 * k * N, sizeof(double) * k * N,
+                           hipMemcpyDeviceToDevice, c->st));
+  RCHK(normalize_dev(c, Z + k * N, k));
+  RCHK(apply_states(c, Z + k * N, HZ + k * N, k, nullptr, nullptr));
+  if (have_prev) {
+    HIPCHK(c, hipMemcpyAsync(Z + 2 * k * N, prev, sizeof(double) * k * N, hipMemcpyDeviceToDevice,
+                             c->st));
+    HIPCHK(c, hipMemcpyAsync(HZ + 2 * k * N, hprev, sizeof(double) * k * N,
+                             hipMemcpyDeviceToDevice, c->st));
+  }
+  // this pass's X and H X become the next pass's previous iterate
+  HIPCHK(c, hipMemcpyAsync(prev, Z, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(hprev, HZ, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
+  // the three Gram matrices
+  RCHK(ensure_scr(c, 3 * (size_t)nz * nz));
+  double *G_d = c->scr, *P_d = G_d + nz * nz, *Q_d = P_d + nz * nz;
+  RCHK(sub_gram_sym(c, Z, Z, nz, 1.0, G_d));
+  RCHK(sub_gram_sym(c, Z, HZ, nz, 1.0, P_d));
+  RCHK(sub_gram_sym(c, HZ, HZ, nz, 1.0, Q_d));
+  std::vector<double> P((size_t)nz * nz), Q((size_t)nz * nz), w(nz);
+  HIPCHK(c, hipMemcpyAsync(P.data(), P_d, sizeof(double) * nz * nz, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(Q.data(), Q_d, sizeof(double) * nz * nz, hipMemcpyDeviceToHost, c->st));
+  RCHK(dev_symeig(c, nz, G_d, w.data(), "refine: Gram"));  // column j of G_d: vector j
+  std::vector<double> W((size_t)nz * nz);
+  HIPCHK(c, hipMemcpyAsync(W.data(), G_d, sizeof(double) * nz * nz, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const double lmax = w[nz - 1], floor2 = SEP_FP32_SVD_FLOOR * SEP_FP32_SVD_FLOOR;
+  std::vector<long> dir;
+  for (long i = nz - 1; i >= 0; i--)
+    if (w[i] > floor2 * lmax) dir.push_back(i);
+  const long r = (long)dir.size();
+  if (r < k) return fail(c, SEP_ERR_NUMERIC, "refine: basis of %ld < %ld states", r, k);
+  // B = W_r L_r^-1/2 (nz x r), reduced H = B^T P B (r x r)
+  std::vector<double> B((size_t)nz * r), PB((size_t)nz * r), Hr((size_t)r * r);
+  for (long j = 0; j < r; j++) {
+    const double sc = 1.0 / sqrt(w[dir[j]]);
+    for (long i = 0; i < nz; i++) B[i * r + j] = W[dir[j] * nz + i] * sc;
+  }
+  // products with unit-stride inner loops (rows of the row-major factors)
+  for (long i = 0; i < nz; i++) {
+    double *out = &PB[i * r];
+    for (long l = 0; l < nz; l++) {
+      const double a = P[i * nz + l], *b = &B[l * r];
+      for (long j = 0; j < r; j++) out[j] += a * b[j];
+    }
+  }
+  for (long l = 0; l < nz; l++) {
+    const double *b = &B[l * r], *pb = &PB[l * r];
+    for (long i = 0; i < r; i++) {
+      const double bi = b[i];
+      double *h = &Hr[i * r];
+      for (long j = 0; j < r; j++) h[j] += bi * pb[j];
+    }
+  }
+  for (long i = 0; i < r; i++)
+    for (long j = 0; j < i; j++) Hr[i * r + j] = Hr[j * r + i] = 0.5 * (Hr[i * r + j] + Hr[j * r + i]);
+  std::vector<double> e(r);
+  RCHK(ensure_scr(c, (size_t)r * r));
+  HIPCHK(c, hipMemcpyAsync(c->scr, Hr.data(), sizeof(double) * r * r, hipMemcpyHostToDevice, c->st));
+  RCHK(dev_symeig(c, r, c->scr, e.data(), "refine: reduced H"));
+  std::vector<double> V((size_t)r * r);
+  HIPCHK(c, hipMemcpyAsync(V.data(), c->scr, sizeof(double) * r * r, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  // C = B V (nz x r): coefficients of the Ritz vectors over Z; their variance
+  // (V column-major: vector j at V[j * r ..]; C[i][j] = sum_l B[i][l] V[j][l])
+  std::vector<double> C((size_t)nz * r), QC((size_t)nz * r), s(r);
+  for (long i = 0; i < nz; i++) {
+    const double *b = &B[i * r];
+    for (long j = 0; j < r; j++) {
+      const double *v = &V[j * r];
+      double acc = 0.0;
+      for (long l = 0; l < r; l++) acc += b[l] * v[l];
+      C[i * r + j] = acc;
+    }
+  }
+  for (long i = 0; i < nz; i++) {
+    double *out = &QC[i * r];
+    for (long l = 0; l < nz; l++) {
+      const double q = Q[i * nz + l], *cl = &C[l * r];
+      for (long j = 0; j < r; j++) out[j] += q * cl[j];
+    }
+  }
+  for (long j = 0; j < r; j++) s[j] = 0.0;
+  for (long i = 0; i < nz; i++)
+    for (long j = 0; j < r; j++) s[j] += C[i * r + j] * QC[i * r + j];
+  for (long j = 0; j < r; j++) s[j] = sqrt(fabs(s[j] - e[j] * e[j]));
+  std::vector<long> idx(r);
+  for (long i = 0; i < r; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](long x, long y) {
+    const bool ix = e[x] >= lo && e[x] <= hi, iy = e[y] >= lo && e[y] <= hi;
+    if (ix != iy) return ix;
+    return s[x] < s[y];
+  });
+  idx.resize(k);
+  std::sort(idx.begin(), idx.end());
+  // the kept Ritz vectors and their H psi, normalized (Ritz vectors are
+  // orthonormal in the plain product: scale 1 / sqrt(dv))
+  const double nrm = 1.0 / sqrt(c->dv);
+  std::vector<double> M((size_t)nz * k);
+  for (long i = 0; i < nz; i++)
+    for (long j = 0; j < k; j++) M[i * k + j] = C[i * r + idx[j]] * nrm;
+  for (long j = 0; j < k; j++) {
+    ev[j] = e[idx[j]];
+    sg[j] = s[idx[j]];
+  }
+  RCHK(ensure_scr(c, (size_t)nz * k));
+  HIPCHK(c, hipMemcpyAsync(c->scr, M.data(), sizeof(double) * nz * k, hipMemcpyHostToDevice, c->st));
+  RCHK(sub_combine(c, Z, nz, c->scr, k, k, Z + 3 * k * N));
+  RCHK(sub_combine(c, HZ, nz, c->scr, k, k, HZ + 3 * k * N));
+  HIPCHK(c, hipMemcpyAsync(Z, Z + 3 * k * N, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(HZ, HZ + 3 * k * N, sizeof(double) * k * N, hipMemcpyDeviceToDevice,
+                           c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));  // M outlives the copy
+  return SEP_OK;
+}
+
+extern "C" int sep_refine_window(sep_ctx *c, long m, double *eval, double *sige, double lo,
+                                 double hi, int passes, long *kw) {
+  if (!c || !eval || !sige || !kw || m < 1 || m > c->cap1 || !(lo < hi) || passes < 1)
+    return SEP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const long N = c->gd.N;
+  // the window's pairs to the front, in order (idx[j] >= j: in place)
+  std::vector<long> idx;
+  for (long i = 0; i < m; i++)
+    if (eval[i] >= lo && eval[i] <= hi) idx.push_back(i);
+  const long k = (long)idx.size();
+  *kw = k;
+  if (k == 0) return SEP_OK;
+  // Z (3k rows) and a k-row output block after it, in both state buffers
+  RCHK(grow_states(c, 4 * k, 1, m));
+  RCHK(grow_states(c, 4 * k, 2, 0));
+  std::vector<double> ev(k), sg(k);
+  for (long j = 0; j < k; j++) {
+    ev[j] = eval[idx[j]];
     if (idx[j] != j)
       HIPCHK(c, hipMemcpyAsync(c->states + j * N, c->states + idx[j] * N, sizeof(double) * N,
                                hipMemcpyDeviceToDevice, c->st));
   }
-  for (long j = keep; j < m; j++) eval[j] = sige[j] = 0.0;
-  HIPCHK(c, hipStreamSynchronize(c->st));
-  return keep == m ? SEP_OK : fail(c, SEP_ERR_NUMERIC, "refine: basis of %ld < %ld states", keep, m);
+  RCHK(apply_states(c, c->states, c->states2, k, nullptr, nullptr));  // H X
+  double *prev = nullptr;
+  HIPCHK(c, hipMalloc(&prev, sizeof(double) * 2 * k * N));
+  int rc = SEP_OK;
+  for (int p = 0; p < passes && rc == SEP_OK; p++)
+    rc = window_pass(c, k, ev.data(), sg.data(), lo, hi, prev, prev + k * N, p > 0);
+  hipFree(prev);
+  RCHK(rc);
+  for (long j = 0; j < m; j++) eval[j] = sige[j] = 0.0;
+  for (long j = 0; j < k; j++) {
+    eval[j] = ev[j];
+    sige[j] = sg[j];
+  }
+  return SEP_OK;
 }
 
 // ------------------------------------------------- Hellmann-Feynman adjoint
