@@ -992,15 +992,49 @@ static int filter_batch_impl(int nd, sep_ctx *const *ctxs, const void *const *ps
         rc = fail(ctxs[d], SEP_ERR_HIP, "batch: graph launch failed");
   } else if (rc == SEP_OK) {
     // interleave: one Newton step of every unfinished grid per round, each
-    // grid on its own stream, so that small grids fill what large ones leave
+    // grid on its own stream(s) -- split into filter lanes as sep_filter_dev
+    // splits it -- so that small grids fill what large ones leave
+    std::vector<std::vector<FilterRun<R>>> lanes(nd);
+    for (int d = 0; d < nd && rc == SEP_OK; d++) {
+      sep_ctx *c = ctxs[d];
+      const int L = filter_lanes<R>(c, runs[d]);
+      if (L <= 1) {
+        lanes[d].assign(1, runs[d]);
+        continue;
+      }
+      const FilterRun<R> &f = runs[d];
+      const long ss = f.K * f.N;
+      if (hipEventRecord(c->lev[0], c->st) != hipSuccess) rc = fail(c, SEP_ERR_HIP, "batch: fork failed");
+      lanes[d].assign((size_t)L, f);
+      for (int l = 0; l < L && rc == SEP_OK; l++) {
+        const long v0 = l * f.nvec / L, v1 = (l + 1) * f.nvec / L;
+        FilterRun<R> &g = lanes[d][l];
+        g.nvec = v1 - v0;
+        g.ns = std::min(2 * v1, f.ns) - 2 * v0;
+        g.ring = f.ring + 2 * v0 * ss;
+        g.tb = f.tb + v0 * f.N;
+        g.acc = f.acc + 2 * v0 * f.ms * f.N;
+        g.st = l == 0 ? c->st : c->lst[l - 1];
+        if (l > 0 && hipStreamWaitEvent(g.st, c->lev[0], 0) != hipSuccess)
+          rc = fail(c, SEP_ERR_HIP, "batch: fork failed");
+      }
+    }
     bool left = true;
     while (rc == SEP_OK && left) {
       left = false;
       for (int d = 0; d < nd && rc == SEP_OK; d++)
-        if (runs[d].j < nc[d]) {
-          rc = filter_step<R>(runs[d]);
+        if (lanes[d][0].j < nc[d]) {
+          for (size_t l = 0; l < lanes[d].size() && rc == SEP_OK; l++)
+            rc = filter_step<R>(lanes[d][l]);
           left = true;
         }
+    }
+    for (int d = 0; d < nd; d++) {  // join each grid's lanes into its stream
+      sep_ctx *c = ctxs[d];
+      for (size_t l = 1; l < lanes[d].size(); l++) {
+        hipEventRecord(c->lev[l], lanes[d][l].st);
+        hipStreamWaitEvent(c->st, c->lev[l], 0);
+      }
     }
   }
   int rc2 = SEP_OK;
