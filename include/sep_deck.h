@@ -54,7 +54,9 @@ typedef struct {
 } sep_deck;
 
 /* Reads the deck in `dir`.  Returns SEP_OK or SEP_ERR_IO / SEP_ERR_ARG with a
- * message in err. */
+ * message in err; a refused deck holds no allocation (sep_deck_free on it
+ * is harmless).  A pot table longer than SEP_NPOT rows is refused (the
+ * reference, filter/read.c:90-92, would write past its table slot). */
 int sep_deck_read(const char *dir, sep_deck *d, char *err, int errlen);
 void sep_deck_free(sep_deck *d);
 /* Symbol of atom type t (filter/read.c:155-176). */

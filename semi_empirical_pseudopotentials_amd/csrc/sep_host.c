@@ -264,7 +264,15 @@ void sep_deck_free(sep_deck *d) {
   memset(d, 0, sizeof *d);
 }
 
+static int deck_read(const char *dir, sep_deck *d, char *err, int errlen);
+
 int sep_deck_read(const char *dir, sep_deck *d, char *err, int errlen) {
+  const int rc = deck_read(dir, d, err, errlen);
+  if (rc != SEP_OK) sep_deck_free(d);  // a refused deck holds nothing
+  return rc;
+}
+
+static int deck_read(const char *dir, sep_deck *d, char *err, int errlen) {
   memset(d, 0, sizeof *d);
   FILE *f = open_rel(dir, "input.par");
   if (!f) f = open_rel(dir, "filter_input.par");
@@ -376,7 +384,15 @@ int sep_deck_read(const char *dir, sep_deck *d, char *err, int errlen) {
     if (f) {
       long k = 0;
       while (k < SEP_NPOT && fscanf(f, "%lg %lg", &vr[k], &pot[k]) == 2) k++;
+      double extra[2];
+      // more rows than the reference's table slot holds (filter/read.c:90-92
+      // would write them into the next type's table): refused, not truncated
+      const int over = k == SEP_NPOT && fscanf(f, "%lg %lg", &extra[0], &extra[1]) == 2;
       fclose(f);
+      if (over) {
+        snprintf(err, errlen, "%s: more than %d rows", name, SEP_NPOT);
+        return SEP_ERR_IO;
+      }
       d->npot[j] = k;
     } else {
       d->npot[j] = d->npot[0];
