@@ -114,6 +114,11 @@ int sep_apply_h_dev_noalloc(sep_ctx *ctx, const void *in_dev, void *out_dev, lon
  * dE = 1.1 (E - vmin) (energy.c:76), the converged E and the iteration count.
  * Replaces get_energy_range(), filter/energy.c:53-83. */
 int sep_energy_range(sep_ctx *ctx, double vmin, double *dE, double *emax, long *iters);
+/* sep_energy_range with the iteration trace of filter/energy.c:65-74
+ * (ene-init.dat): log[3 i .. 3 i + 2] = (previous energy, energy, norm of
+ * H psi) of iteration i, for i < min(iters, maxlog). */
+int sep_energy_range_log(sep_ctx *ctx, double vmin, double *dE, double *emax, long *iters,
+                         double *log, long maxlog);
 
 /* ------------------------------------------------- random states (host) */
 /* Numerical Recipes ran2 state that filter/rand.c:28-66 keeps in statics. */

@@ -103,6 +103,7 @@ SIGNATURES = {
     "sep_set_filter_lanes": (_I, [_V, _I]),
     "sep_set_hpsi_family": (_I, [_V, _I, _I]),
     "sep_so_setup": (_I, [_V, _L, _dp, _dp, _lp]),
+    "sep_energy_range_log": (_I, [_V, _D, _dp, _dp, _lp, _dp, _L]),
     "sep_refine_window": (_I, [_V, _L, _dp, _dp, _D, _D, _I, _lp]),
     "sep_apply_h_spinor_dev": (_I, [_V, _V, _V, _L]),
     "sep_energy_range_spinor": (_I, [_V, _dp, _dp, _lp]),

@@ -570,7 +570,12 @@ int sep_apply_h_dev_noalloc(sep_ctx *c, const void *in_dev, void *out_dev, long 
 
 // ----------------------------------------------------------- power iteration
 int sep_energy_range(sep_ctx *c, double vmin, double *dE, double *emax, long *iters) {
-  if (!c) return SEP_ERR_ARG;
+  return sep_energy_range_log(c, vmin, dE, emax, iters, nullptr, 0);
+}
+
+int sep_energy_range_log(sep_ctx *c, double vmin, double *dE, double *emax, long *iters,
+                         double *log, long maxlog) {
+  if (!c || (maxlog > 0 && !log)) return SEP_ERR_ARG;
   if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
   HIPCHK(c, hipSetDevice(c->device));
   const long N = c->gd.N;
@@ -613,6 +618,11 @@ int sep_energy_range(sep_ctx *c, double vmin, double *dE, double *emax, long *it
     ene_old = ene;
     if ((rc = apply_views<double>(c, V, W, 1, 1.0, 0.0, &sums)) != SEP_OK) break;
     ene = (sums[0] + sums[1]) * c->dv;
+    if (i < maxlog) {  // the line of ene-init.dat (filter/energy.c:72)
+      log[3 * i] = ene_old;
+      log[3 * i + 1] = ene;
+      log[3 * i + 2] = norma;
+    }
   }
   hipFree(v);
   hipFree(w);

@@ -3,7 +3,9 @@
  * reference's filter.x (filter/main.c:11-162).  Reads input.par, conf.par,
  * pot*.par (+ optional expot.par, strain.par, seed.par) from the working
  * directory (or --dir), writes eval.dat, psi.dat and the reference's side
- * files eval-filt-0-<jns>.dat, Egrid.dat, func.dat.
+ * files eval-filt-0-<jns>.dat, Egrid.dat, func.dat, coeff.dat, conf.dat,
+ * pot<Sym>.dat, pS.dat, localPot.cube, ene-init.dat, prop0.dat, hmat.dat
+ * (psi-filt-0-<jns>.dat with --psi-filt).
  *
  * Options (the reference ignores argv; these are extensions):
  *   --dir D            deck / output directory (default .)
@@ -82,6 +84,104 @@ static void write_func(const double *an0, const double *zn, long nc, double dE, 
     fprintf(pf, "%g %g %g\n", xu, f, g);
   }
   fclose(pf);
+}
+
+/* conf.dat (filter/read.c:32-36): recentred positions and type index. */
+static void write_conf(const sep_deck *d) {
+  FILE *pw = out_file("conf.dat", "w");
+  if (!pw) return;
+  for (long i = 0; i < d->natom; i++)
+    fprintf(pw, "%s %g %g %g %d\n", d->sym[i], d->xyz[3 * i], d->xyz[3 * i + 1],
+            d->xyz[3 * i + 2], d->type[i]);
+  fclose(pw);
+}
+
+/* pot<Sym>.dat (filter/read.c:108-116): every type's shifted table. */
+static void write_pots(const sep_deck *d) {
+  for (int j = 0; j < SEP_NTYPE; j++) {
+    char name[32];
+    snprintf(name, sizeof name, "pot%s.dat", sep_type_symbol(j));
+    FILE *pf = out_file(name, "w");
+    if (!pf) continue;
+    for (long k = 0; k < d->npot[j]; k++)
+      fprintf(pf, "%g %g\n", d->vr[j * SEP_NPOT + k], d->pot[j * SEP_NPOT + k]);
+    fclose(pf);
+  }
+}
+
+/* pS.dat (filter/init.c:185-190): the external potential along z,
+ * e (1 / (exp((|z| - x0) / kT) + 1) - 1) (init.c:255-257). */
+static void write_ps(const sep_deck *d) {
+  FILE *pf = out_file("pS.dat", "w");
+  if (!pf) return;
+  double z = d->zmin;
+  for (long jz = 0; jz < d->nz; jz++, z += d->dz) {
+    const double e = d->expot[0] * (1.0 / (exp((fabs(z) - d->expot[1]) / d->expot[2]) + 1.0) - 1.0);
+    fprintf(pf, "% .6f % .8f\n", z, e);
+  }
+  fclose(pf);
+}
+
+/* localPot.cube (filter/write.c:31-119, called at init.c:192): header,
+ * the atoms of conf.par as read there (2-character symbols, positions as in
+ * the file), then V with the reference's index nz (ny iX + iY) + iZ. */
+static void write_cube(const sep_deck *d, const double *V) {
+  char path[4096];
+  snprintf(path, sizeof path, "%s/conf.par", g_dir);
+  FILE *pc = fopen(path, "r");
+  FILE *pf = out_file("localPot.cube", "w");
+  if (!pc || !pf) {
+    if (pc) fclose(pc);
+    if (pf) fclose(pf);
+    return;
+  }
+  long na = 0;
+  if (fscanf(pc, "%ld", &na) != 1) na = 0;
+  fprintf(pf, "CUBE FILE\n");
+  fprintf(pf, "OUTER LOOP: X, MIDDLE LOOP: Y, INNER LOOP: Z\n");
+  fprintf(pf, "%5li%12.6f%12.6f%12.6f\n", na, d->xmin, d->ymin, d->zmin);
+  fprintf(pf, "%5li%12.6f%12.6f%12.6f\n", d->nx, d->dx, 0.0, 0.0);
+  fprintf(pf, "%5li%12.6f%12.6f%12.6f\n", d->ny, 0.0, d->dy, 0.0);
+  fprintf(pf, "%5li%12.6f%12.6f%12.6f\n", d->nz, 0.0, 0.0, d->dz);
+  static const struct { const char *s; int z; } Z[] = {
+      {"Cd", 48}, {"S", 16}, {"Se", 34}, {"Zn", 30}, {"Te", 52}, {"C", 6}, {"Si", 14},
+      {"Cs", 55}, {"Pb", 82}, {"I", 53}, {"In", 49}, {"As", 33}, {"Ga", 31}, {"P", 15}};
+  char line[80], sym[10];
+  double x, y, z;
+  if (!fgets(line, sizeof line, pc)) line[0] = 0;
+  while (fgets(line, sizeof line, pc)) {
+    if (sscanf(line, "%2s %lf %lf %lf", sym, &x, &y, &z) != 4) continue;
+    int at = 1;
+    for (size_t k = 0; k < sizeof Z / sizeof Z[0]; k++)
+      if (!strcmp(sym, Z[k].s)) at = Z[k].z;
+    fprintf(pf, "%5i%12.6f%12.6f%12.6f%12.6f\n", at, 0.0, x, y, z);
+  }
+  for (long ix = 0; ix < d->nx; ix++)
+    for (long iy = 0; iy < d->ny; iy++) {
+      const long ixy = d->nz * (d->ny * ix + iy);
+      for (long iz = 0; iz < d->nz; iz++) {
+        fprintf(pf, "%g ", V[ixy + iz]);
+        if (iz % 6 == 5) fprintf(pf, "\n");
+      }
+      fprintf(pf, "\n");
+    }
+  fclose(pc);
+  fclose(pf);
+}
+
+/* hmat.dat (filter/Hmat.c:28-80): the progress lines of the reference's
+ * Rayleigh-Ritz (its rows, its eigensolve, its rotation of every 1000th
+ * grid point). */
+static void write_hmat_log(long mstot, long ngrid) {
+  FILE *pg = out_file("hmat.dat", "w");
+  if (!pg) return;
+  fprintf(pg, "calculate the H matrix\n");
+  for (long i = 0; i < mstot; i++) fprintf(pg, "finshed row %ld\n", i);
+  fprintf(pg, "diagonalize the new Hamiltonian H\n");
+  fprintf(pg, "copy the new function into psitot\n");
+  for (long g = 0; g < ngrid; g += 1000) fprintf(pg, "finished grid point %ld\n", g);
+  fprintf(pg, "free memory\n");
+  fclose(pg);
 }
 
 int main(int argc, char **argv) {
@@ -169,9 +269,16 @@ int main(int argc, char **argv) {
   int *ti = (int *)malloc(sizeof(int) * d.natom);
   for (long i = 0; i < d.natom; i++) ti[i] = d.type[i];
   double Vmin, Vmax;
+  double *Vhost = (double *)malloc(sizeof(double) * N);
+  if (!Vhost) die("potential buffer");
   CHECK(sep_build_potential(ctx, d.natom, d.xyz, ti, d.has_strain ? d.scale : NULL, SEP_NTYPE,
-                            d.npot, d.vr, d.pot, SEP_NPOT, d.dr, d.expot, NULL, &Vmin, &Vmax));
+                            d.npot, d.vr, d.pot, SEP_NPOT, d.dr, d.expot, Vhost, &Vmin, &Vmax));
   free(ti);
+  write_conf(&d);
+  write_pots(&d);
+  write_ps(&d);
+  write_cube(&d, Vhost);
+  free(Vhost);
   printf("Potential energy statistics: dV = %.5f Vmin = % .5f Vmax = %.5f\n", Vmax - Vmin, Vmin,
          Vmax);
   const double PIE = 3.14159265358979323846;
@@ -192,7 +299,17 @@ int main(int argc, char **argv) {
   double t0 = wall(), c0 = (double)clock();
   double dE, Emax;
   long iters;
-  CHECK(sep_energy_range(ctx, Vmin, &dE, &Emax, &iters));
+  {
+    double *elog = (double *)malloc(sizeof(double) * 3 * 500);
+    CHECK(sep_energy_range_log(ctx, Vmin, &dE, &Emax, &iters, elog, 500));
+    FILE *pf = out_file("ene-init.dat", "w");  /* filter/energy.c:65-74 */
+    if (pf) {
+      for (long i = 0; i < iters && i < 500; i++)
+        fprintf(pf, "%ld %.16g %.16g %.16g\n", i, elog[3 * i], elog[3 * i + 1], elog[3 * i + 2]);
+      fclose(pf);
+    }
+    free(elog);
+  }
   printf("dE = %g\n", dE);
   printf("done calculate energy range, CPU time (sec) %g, wall run time (sec) %g\n",
          ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
@@ -251,6 +368,15 @@ int main(int argc, char **argv) {
     c0 = (double)clock();
     CHECK(sep_filter(ctx, psi0, ns, an, zn, nc, ms, Vmin, dE, prec, psif, ene));
     CHECK(sep_trim(ctx)); /* one filter per run: its ring is not needed again */
+    {
+      /* prop0.dat (filter/filter.c:69-81): every 100th Newton step; the file
+       * is rewritten per random state, the last one remains */
+      FILE *pf = out_file("prop0.dat", "w");
+      if (pf) {
+        for (long j = 100; j < nc; j += 100) fprintf(pf, "%ld %ld\n", j, ns - 1);
+        fclose(pf);
+      }
+    }
     printf("done calculating filter, CPU time (sec) %g, wall run time (sec) %g\n",
            ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
     for (long jns = 0; jns < ns; jns++) { /* filter.c:38-48 */
@@ -291,6 +417,7 @@ int main(int argc, char **argv) {
   double *eval = (double *)malloc(sizeof(double) * kept);
   double *sige = (double *)malloc(sizeof(double) * kept);
   CHECK(sep_rayleigh_ritz(ctx, kept, eval));
+  write_hmat_log(kept, N);
   if (prec == SEP_PREC_FP32) CHECK(sep_refine(ctx, kept, eval, sige));
   printf("done calculating Hmat, CPU time (sec) %g, wall run time (sec) %g\n",
          ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
