@@ -310,13 +310,17 @@ def main():
     nvec = (ns + 1) // 2
     if args.kernels_only:
         k = ctx.bench_hpsi(sep.SEP_PREC_FP32, nvec, ms, 5)
+        # the fused kernel's previous (xor-swizzled LDS) form, for comparison
+        ctx.set_hpsi_family(sep.SEP_PREC_FP32, sep.SEP_HPSI_PLANE_SWIZZLED)
+        k_swz = ctx.bench_hpsi(sep.SEP_PREC_FP32, nvec, ms, 5)
+        ctx.set_hpsi_family(sep.SEP_PREC_FP32, sep.SEP_HPSI_PLANE)
         a = torch.empty(64 << 20, dtype=torch.float32, device="cuda:0").fill_(1.0)
         b = torch.empty_like(a)
         for _ in range(3):
             torch.add(a, 1.0, out=b)  # plain copy kernel: 256 MiB read + 256 MiB written
         torch.cuda.synchronize()
-        print(json.dumps({"kernels_only": True, "pass_ms": list(k), "copy_bytes": 2 * a.numel() * 4}),
-              flush=True)
+        print(json.dumps({"kernels_only": True, "pass_ms": list(k), "pass_ms_swizzled": list(k_swz),
+                          "copy_bytes": 2 * a.numel() * 4}), flush=True)
         ctx.close()
         return
     # ------------------------------------------------------------ timed loop
@@ -453,8 +457,12 @@ def main():
 
     r32 = roof(live32, 8, 4)
     dom = max((q for q in r32 if not q.startswith("step")), key=lambda q: r32[q]["ms"])
+    fused = {sep.SEP_HPSI_PLANE: "k_plane_bwd_fwd_s<1024>",
+             sep.SEP_HPSI_PLANE_SWIZZLED: "k_plane_bwd_fwd_w<float,128>",
+             sep.SEP_HPSI_PLANE_CHUNKED: "k_plane_bwd_fwd_h<float,128>"}.get(
+                 ctx.hpsi_family(sep.SEP_PREC_FP32), "k_rows_bwd_fwd<float,128>")
     kname = {"A": "k_plane_fwd_w<float,128>", "B": "k_pencil<float,128>",
-             "C": "k_plane_bwd_w<float,128>", "CA": "k_plane_bwd_fwd_w<float,128>"}[dom]
+             "C": "k_plane_bwd_w<float,128>", "CA": fused}[dom]
     traffic = measured_traffic(kname, n, nvec)
     probe = round(ctx.bench_copy(4 << 30, 5), 1)  # plain 16 B/lane copy, best of 4 grids
     roofline = {"bound": "hbm", "kernel": kname, "achieved": r32[dom]["GBs"],

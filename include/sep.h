@@ -258,11 +258,15 @@ int sep_autotune(sep_ctx *ctx, int precision, long nvec, double *ms_plane, doubl
 #define SEP_HPSI_PLANE_CHUNKED 1 /* same, the fused kernel transposing through half a \
                                     plane of LDS so that two workgroups share a CU */
 #define SEP_HPSI_LINES 2         /* row + strided-line kernels */
-#define SEP_HPSI_NFAMILY 3
+#define SEP_HPSI_PLANE_SWIZZLED 3 /* whole-plane kernels, the fused kernel on the \
+                                     xor-swizzled LDS placement that SEP_HPSI_PLANE \
+                                     replaces at complex64 128 x 128 */
+#define SEP_HPSI_NFAMILY 4
 /* Selects the H.psi kernel family of this context in `precision`.
  * SEP_ERR_UNSUPPORTED if the grid has no such form (plane families need
- * nx == ny and a plane length; the chunked form exists for complex64
- * 128 x 128 planes).  All families give the same H.psi to rounding. */
+ * nx == ny and a plane length; the chunked and swizzled forms exist for
+ * complex64 128 x 128 planes).  All families give the same H.psi to
+ * rounding. */
 int sep_set_hpsi_family(sep_ctx *ctx, int precision, int family);
 int sep_get_hpsi_family(sep_ctx *ctx, int precision, int *family);
 /* Autotuner over every family this grid has and over the number of filter

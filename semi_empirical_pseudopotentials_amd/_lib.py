@@ -20,7 +20,8 @@ SEP_INTERP_CUBIC = 1
 SEP_HPSI_PLANE = 0
 SEP_HPSI_PLANE_CHUNKED = 1
 SEP_HPSI_LINES = 2
-SEP_HPSI_NFAMILY = 3
+SEP_HPSI_PLANE_SWIZZLED = 3
+SEP_HPSI_NFAMILY = 4
 SEP_PREC_FP64 = 0
 SEP_PREC_FP32 = 1
 _ERRNAMES = {1: "SEP_ERR_ARG", 2: "SEP_ERR_UNSUPPORTED", 3: "SEP_ERR_HIP", 4: "SEP_ERR_IO",
@@ -409,7 +410,8 @@ class Context:
         return float(tp[0]), float(tl[0])
 
     def set_hpsi_family(self, precision, family):
-        """H.psi kernel family (SEP_HPSI_PLANE / _PLANE_CHUNKED / _LINES) for this grid."""
+        """H.psi kernel family (SEP_HPSI_PLANE / _PLANE_CHUNKED / _LINES / _PLANE_SWIZZLED)
+        for this grid."""
         self._chk(lib().sep_set_hpsi_family(self._h, precision, family), "sep_set_hpsi_family")
 
     def hpsi_family(self, precision):

@@ -22,6 +22,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <type_traits>
 
 namespace sep {
@@ -248,14 +249,14 @@ struct Pass {
 
 // Index read by element (b, k) of thread g in pass p.
 template <int N, int S, int p>
-__device__ __forceinline__ int in_idx(int g, int b, int k) {
+__host__ __device__ __forceinline__ constexpr int in_idx(int g, int b, int k) {
   using PL = FftPlan<N>;
   constexpr int r = Pass<N, S>::radix(p);
   return g + PL::G * b + k * (N / r);
 }
 // Index written by element (b, k) of thread g in pass p.
 template <int N, int S, int p>
-__device__ __forceinline__ int out_idx(int g, int b, int k) {
+__host__ __device__ __forceinline__ constexpr int out_idx(int g, int b, int k) {
   using PL = FftPlan<N>;
   constexpr int r = Pass<N, S>::radix(p);
   constexpr int Ns = Pass<N, S>::ns(p);
@@ -454,6 +455,151 @@ __device__ __forceinline__ void run_passes_c(cx<R> *v, int g, bool act, const Tw
     }
     run_passes_c<N, S, p + 1, pend, Sync>(v, g, act, tc, at);
   }
+}
+
+// ------------------------------------------------- separable line layouts
+// A line layout F maps a line index i to an offset F::f(i) that is additive
+// over disjoint bits, f(a + b) = f(a) + f(b) when a & b == 0 (a weighted sum
+// of the index bits).  In every pass of a power-of-two Stockham plan the
+// index of element (b, k) of thread g is the bitwise-disjoint sum of a thread
+// part (g, 0, 0) and an element part (0, b, k), so its offset is a per-thread
+// base plus a compile-time constant: one address per thread and pattern, the
+// element offsets folded into the LDS instructions' immediate field.
+template <int N, int S, int p, bool OUT, typename F>
+constexpr bool sep_pass_ok() {
+  using PL = FftPlan<N>;
+  constexpr int r = Pass<N, S>::radix(p);
+  for (int g = 0; g < PL::G; g++)
+    for (int b = 0; b < PL::E / r; b++)
+      for (int k = 0; k < r; k++) {
+        const int i = OUT ? out_idx<N, S, p>(g, b, k) : in_idx<N, S, p>(g, b, k);
+        const int t = OUT ? out_idx<N, S, p>(g, 0, 0) : in_idx<N, S, p>(g, 0, 0);
+        const int c = OUT ? out_idx<N, S, p>(0, b, k) : in_idx<N, S, p>(0, b, k);
+        if ((t & c) != 0 || t + c != i || F::f(i) != F::f(t) + F::f(c)) return false;
+      }
+  return true;
+}
+template <int N, int S, typename F, int p = 0>
+constexpr bool sep_plan_ok() {
+  if constexpr (p == FftPlan<N>::P) {
+    return true;
+  } else {
+    return sep_pass_ok<N, S, p, false, F>() && sep_pass_ok<N, S, p, true, F>() &&
+           sep_plan_ok<N, S, F, p + 1>();
+  }
+}
+
+// Keeps the compiler from pairing the 8-byte LDS accesses on either side
+// into ds_read2_b64 / ds_write2_b64: on gfx950 a single 8-byte access serves
+// a half-wave per cycle (64 banks of 4 B; 2 cycles a wave), a paired one
+// measured 8 cycles a wave whatever its two offsets, twice the cost of the
+// two single accesses.  No instruction is emitted.
+__device__ __forceinline__ void lds_unpaired() { asm volatile("" ::: "memory"); }
+
+// write_out / read_in on a line at `line` laid out by F.
+template <int N, int S, int p, typename F, typename R>
+__device__ __forceinline__ void write_out_sep(const cx<R> *v, int g, cx<R> *line) {
+  using PL = FftPlan<N>;
+  constexpr int r = Pass<N, S>::radix(p);
+  cx<R> *t = line + F::f(out_idx<N, S, p>(g, 0, 0));
+#pragma unroll
+  for (int b = 0; b < PL::E / r; b++)
+#pragma unroll
+    for (int k = 0; k < r; k++) {
+      t[F::f(out_idx<N, S, p>(0, b, k))] = v[b * r + k];
+      lds_unpaired();
+    }
+}
+template <int N, int S, int p, typename F, typename R>
+__device__ __forceinline__ void read_in_sep(cx<R> *v, int g, const cx<R> *line) {
+  using PL = FftPlan<N>;
+  constexpr int r = Pass<N, S>::radix(p);
+  const cx<R> *t = line + F::f(in_idx<N, S, p>(g, 0, 0));
+#pragma unroll
+  for (int b = 0; b < PL::E / r; b++)
+#pragma unroll
+    for (int k = 0; k < r; k++) {
+      v[b * r + k] = t[F::f(in_idx<N, S, p>(0, b, k))];
+      lds_unpaired();
+    }
+}
+
+// Twiddles of passes 1..P-1 in a per-thread-index LDS table: thread g's
+// factors (TwCache's order) at tab[g * SLOTS + slot], so that every twiddle
+// read is the thread's base plus a compile-time slot.  host_table() builds
+// the table (the values of the exp(-2 pi i m / N) table rounded to R, the
+// imaginary part negated for S > 0, as TwTable reads them).
+template <int N, int S, typename R>
+struct TwSep {
+  using PL = FftPlan<N>;
+  static constexpr int SLOTS = TwLayout<N, S>::total;
+  static constexpr int WORDS = SLOTS * PL::G;
+  const cx<R> *base;
+  static void host_table(cx<R> *tab) {
+    for (int g = 0; g < PL::G; g++)
+      for (int p = 1; p < PL::P; p++) {
+        const int r = Pass<N, S>::radix(p), Ns = Pass<N, S>::ns(p);
+        for (int b = 0; b < PL::E / r; b++)
+          for (int k = 1; k < r; k++) {
+            const int t = (g + PL::G * b) % Ns;
+            const int m = k * t * (N / (Ns * r));
+            const double ang = -6.283185307179586476925286766559 * (double)m / (double)N;
+            cx<R> x;
+            x.re = (R)std::cos(ang);
+            x.im = (R)std::sin(ang);
+            if (S > 0) x.im = -x.im;
+            tab[g * SLOTS + TwLayout<N, S>::off(p) + b * (r - 1) + (k - 1)] = x;
+          }
+      }
+  }
+  __device__ __forceinline__ void init(int g, const cx<R> *tab) { base = tab + g * SLOTS; }
+};
+
+// The twiddles of pass p of a TwSep source, read into registers together
+// (unpaired accesses issued back to back, one wait for all) and served to
+// compute_pass_c.
+template <int N, int S, int p, typename R>
+struct TwSepPass {
+  static constexpr int r = Pass<N, S>::radix(p);
+  static constexpr int NW = p > 0 ? (FftPlan<N>::E / r) * (r - 1) : 0;
+  cx<R> w[NW > 0 ? NW : 1];
+  __device__ __forceinline__ void load(const TwSep<N, S, R> &t) {
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+      w[i] = t.base[TwLayout<N, S>::off(p) + i];
+      lds_unpaired();
+    }
+  }
+  template <int q>
+  __device__ __forceinline__ cx<R> get(int b, int k) const {
+    return w[b * (r - 1) + (k - 1)];
+  }
+};
+
+// run_passes_c on a line laid out by F with TwSep twiddles: the twiddles of
+// pass p + 1 are read during the exchange after pass p.  tp holds pass p's.
+template <int N, int S, int p, int pend, typename F, typename R>
+__device__ __forceinline__ void run_passes_sep(cx<R> *v, int g, const TwSep<N, S, R> &tc,
+                                               cx<R> *line, const TwSepPass<N, S, p, R> &tp) {
+  if constexpr (p < pend) {
+    compute_pass_c<N, S, p>(v, tp);
+    if constexpr (p + 1 < pend) {
+      WaveSync::sync();
+      write_out_sep<N, S, p, F>(v, g, line);
+      TwSepPass<N, S, p + 1, R> tn;
+      tn.load(tc);
+      WaveSync::sync();
+      read_in_sep<N, S, p + 1, F>(v, g, line);
+      run_passes_sep<N, S, p + 1, pend, F>(v, g, tc, line, tn);
+    }
+  }
+}
+// All passes of one line.
+template <int N, int S, typename F, typename R>
+__device__ __forceinline__ void run_line_sep(cx<R> *v, int g, const TwSep<N, S, R> &tc,
+                                             cx<R> *line) {
+  TwSepPass<N, S, 0, R> t0;  // pass 0 has no twiddles
+  run_passes_sep<N, S, 0, FftPlan<N>::P, F>(v, g, tc, line, t0);
 }
 
 }  // namespace sep

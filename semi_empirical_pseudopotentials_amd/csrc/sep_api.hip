@@ -272,6 +272,15 @@ int sep_open(const sep_grid_desc *g, sep_ctx **out) {
       return SEP_ERR_HIP;
     }
   }
+  if (const long twb = plane_sep_tw_bytes(c->gd.nx, SEP_PREC_FP32)) {
+    std::vector<char> tws(twb);
+    plane_sep_tw_host(tws.data());
+    if (hipMalloc(&c->gd.twsep32, twb) != hipSuccess ||
+        hipMemcpy(c->gd.twsep32, tws.data(), twb, hipMemcpyHostToDevice) != hipSuccess) {
+      sep_close(c);
+      return SEP_ERR_HIP;
+    }
+  }
   c->gd.kxx = tabs_d[0];
   c->gd.kyy = tabs_d[1];
   c->gd.kzz = tabs_d[2];
@@ -289,6 +298,7 @@ int sep_close(sep_ctx *c) {
     if (c->gd.tw64[a]) hipFree(c->gd.tw64[a]);
     if (c->gd.tw32[a]) hipFree(c->gd.tw32[a]);
   }
+  if (c->gd.twsep32) hipFree(c->gd.twsep32);
   for (void *p : {(void *)c->gd.kxx, (void *)c->gd.kyy, (void *)c->gd.kzz, (void *)c->pot64,
                   (void *)c->pot32, (void *)c->states, (void *)c->states2, c->tb, (void *)c->scr,
                   (void *)c->gwork, c->ring})
@@ -2090,7 +2100,8 @@ extern "C" int sep_autotune_ex(sep_ctx *c, int precision, long nvec, double *ms,
     return SEP_ERR_ARG;
   if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
   HIPCHK(c, hipSetDevice(c->device));
-  const bool f_lines = c->gd.force_lines[precision], f_chunk = c->gd.fused_chunked[precision];
+  const bool f_lines = c->gd.force_lines[precision];
+  const int f_form = c->gd.fused_form[precision];
   const int l0 = c->lanes;
   const bool timing = c->timing;
   c->timing = false;  // live timing turns lanes off
@@ -2121,7 +2132,7 @@ extern "C" int sep_autotune_ex(sep_ctx *c, int precision, long nvec, double *ms,
   c->timing = timing;
   if (rc != SEP_OK || bf < 0) {
     c->gd.force_lines[precision] = f_lines;
-    c->gd.fused_chunked[precision] = f_chunk;
+    c->gd.fused_form[precision] = f_form;
     c->lanes = l0;
     return rc != SEP_OK ? rc : fail(c, SEP_ERR_UNSUPPORTED, "autotune: no family ran");
   }
@@ -2139,17 +2150,22 @@ extern "C" int sep_set_hpsi_family(sep_ctx *c, int precision, int family) {
   switch (family) {
     case SEP_HPSI_PLANE:
     case SEP_HPSI_PLANE_CHUNKED:
+    case SEP_HPSI_PLANE_SWIZZLED: {
       if (!plane_supported(c->gd.nx, c->gd.ny, precision))
         return fail(c, SEP_ERR_UNSUPPORTED, "no plane kernels for %d x %d", c->gd.nx, c->gd.ny);
-      if (family == SEP_HPSI_PLANE_CHUNKED && !plane_fused_chunked_supported(c->gd.nx, precision))
-        return fail(c, SEP_ERR_UNSUPPORTED, "no chunked fused kernel for %d x %d in this precision",
+      const int form = family == SEP_HPSI_PLANE_CHUNKED    ? SEP_FUSED_CHUNKED
+                       : family == SEP_HPSI_PLANE_SWIZZLED ? SEP_FUSED_SWIZZLED
+                                                           : SEP_FUSED_PLANE;
+      if (!plane_fused_form_supported(c->gd.nx, precision, form))
+        return fail(c, SEP_ERR_UNSUPPORTED, "no such fused kernel for %d x %d in this precision",
                     c->gd.nx, c->gd.ny);
       c->gd.force_lines[precision] = false;
-      c->gd.fused_chunked[precision] = family == SEP_HPSI_PLANE_CHUNKED;
+      c->gd.fused_form[precision] = form;
       return SEP_OK;
+    }
     case SEP_HPSI_LINES:
       c->gd.force_lines[precision] = true;
-      c->gd.fused_chunked[precision] = false;
+      c->gd.fused_form[precision] = SEP_FUSED_PLANE;
       return SEP_OK;
   }
   return fail(c, SEP_ERR_ARG, "unknown H.psi family %d", family);
@@ -2160,8 +2176,11 @@ extern "C" int sep_get_hpsi_family(sep_ctx *c, int precision, int *family) {
     return SEP_ERR_ARG;
   const bool plane = !c->gd.force_lines[precision] &&
                      plane_supported(c->gd.nx, c->gd.ny, precision);
-  *family = !plane ? SEP_HPSI_LINES
-                   : (c->gd.fused_chunked[precision] ? SEP_HPSI_PLANE_CHUNKED : SEP_HPSI_PLANE);
+  const int form = c->gd.fused_form[precision];
+  *family = !plane                       ? SEP_HPSI_LINES
+            : form == SEP_FUSED_CHUNKED  ? SEP_HPSI_PLANE_CHUNKED
+            : form == SEP_FUSED_SWIZZLED ? SEP_HPSI_PLANE_SWIZZLED
+                                         : SEP_HPSI_PLANE;
   return SEP_OK;
 }
 

@@ -916,6 +916,188 @@ __global__ void __launch_bounds__(T) k_plane_bwd_fwd_w(cx<R> *tb, VecView<R> pin
   }
 }
 
+// Plane placement of the separable-layout fused kernel (complex64, 128^2):
+// element (y, x) at Y::f(y) + X::f(x) (8-byte words), each a weighted sum of
+// the index bits, so every FFT access pattern is a per-thread base plus a
+// compile-time offset (sep_plan_ok).  On gfx950 an 8-byte LDS read serves 32
+// lanes a cycle (bank = word mod 32) and a write 16 (word mod 16).  The
+// weights' residues mod 32 (1, 4, 8, 6, 5, 16, 2 for bits 0..6 of either
+// index) make the 16 lanes of a line write 16 distinct words mod 16 in every
+// pattern of the radix-8/8/2 plan except two (two ways: index bits 3..6 and
+// 0, 4, 5, 6), and the two lines of a half-wave, which differ in bit 5 of
+// their line index (16 mod 32 apart), read 32 distinct words mod 32.  The
+// weights are the smallest superincreasing sequence with those residues
+// (so the placement is one to one): 17429 words, 136 KB.
+struct PlaneSep128Y {
+  __host__ __device__ static constexpr int f(int y) {
+    return 33 * (y & 1) + 132 * ((y >> 1) & 1) + 264 * ((y >> 2) & 1) + 4358 * ((y >> 3) & 1) +
+           1093 * ((y >> 4) & 1) + 8720 * ((y >> 5) & 1) + 66 * ((y >> 6) & 1);
+  }
+};
+struct PlaneSep128X {
+  __host__ __device__ static constexpr int f(int x) {
+    return (x & 1) + 4 * ((x >> 1) & 1) + 8 * ((x >> 2) & 1) + 2182 * ((x >> 3) & 1) +
+           549 * ((x >> 4) & 1) + 16 * ((x >> 5) & 1) + 2 * ((x >> 6) & 1);
+  }
+};
+// Line of thread group tl (0..63) in a round: bits 0 and 5 swapped, so that
+// the two lines of a half-wave (tl = 2m, 2m + 1) differ in bit 5.
+__host__ __device__ constexpr int plane_sep_line(int tl) {
+  return (tl & 30) | ((tl & 1) << 5) | ((tl >> 5) & 1);
+}
+constexpr int kPlaneSep128Words = PlaneSep128Y::f(127) + PlaneSep128X::f(127) + 1;
+constexpr bool plane_sep128_one_to_one() {
+  bool used[kPlaneSep128Words] = {};
+  for (int y = 0; y < 128; y++)
+    for (int x = 0; x < 128; x++) {
+      const int a = PlaneSep128Y::f(y) + PlaneSep128X::f(x);
+      if (used[a]) return false;
+      used[a] = true;
+    }
+  return true;
+}
+static_assert(plane_sep128_one_to_one(), "plane placement must be one to one");
+static_assert(sep_plan_ok<128, 1, PlaneSep128Y>() && sep_plan_ok<128, -1, PlaneSep128Y>() &&
+                  sep_plan_ok<128, 1, PlaneSep128X>() && sep_plan_ok<128, -1, PlaneSep128X>(),
+              "plane placement must be separable for every pass of the 128 plan");
+
+constexpr size_t plane_sep_lds() {
+  return sizeof(cx<float>) *
+         (size_t)(kPlaneSep128Words + TwSep<128, 1, float>::WORDS + TwSep<128, -1, float>::WORDS);
+}
+
+// Per-thread-index twiddle tables of the separable fused kernel (TwSep, the
+// inverse transform's then the forward one's), built on the host.
+constexpr int plane_sep_tw_words() {
+  return TwSep<128, 1, float>::WORDS + TwSep<128, -1, float>::WORDS;
+}
+void plane_sep_tw_host(void *tab) {
+  cx<float> *t = static_cast<cx<float> *>(tab);
+  TwSep<128, 1, float>::host_table(t);
+  TwSep<128, -1, float>::host_table(t + TwSep<128, 1, float>::WORDS);
+}
+long plane_sep_tw_bytes(int n, int prec) {
+  return (n == 128 && prec == SEP_PREC_FP32) ? (long)sizeof(cx<float>) * plane_sep_tw_words() : 0;
+}
+
+// k_plane_bwd_fwd_w for complex64 128^2 on the separable placement: the same
+// arithmetic (pass C of step j, epilogue, pass A of step j+1, plane by plane),
+// every LDS access a thread base plus an immediate offset, the twiddles read
+// from per-thread-index tables (TwSep, copied from tws).  Between the inverse
+// and the forward x transform of a row the data stay in registers: the last
+// inverse pass writes the elements in the order the first forward pass reads
+// them (both g + G b + k N / R[0]).
+template <int T>
+__global__ void __launch_bounds__(T) k_plane_bwd_fwd_s(cx<float> *tb, VecView<float> pin,
+                                                       VecView<float> out, Epi<float> epi, long N,
+                                                       int nz, const cx<float> *tws) {
+  constexpr int NX = 128;
+  using R = float;
+  using PL = FftPlan<NX>;
+  constexpr int G = PL::G, E = PL::E, P = PL::P;
+  using LY = PlaneSep128Y;
+  using LX = PlaneSep128X;
+  constexpr int RX = T / G;
+  constexpr int NR = NX / RX;
+  static_assert(NX % RX == 0, "whole rounds of lines");
+  static_assert(Pass<NX, 1>::radix(P - 1) == Pass<NX, -1>::radix(0),
+                "the inverse transform's last pass is the forward transform's first");
+  extern __shared__ char smem_raw[];
+  cx<R> *buf = reinterpret_cast<cx<R> *>(smem_raw);
+  cx<R> *twb = buf + kPlaneSep128Words;
+  cx<R> *twf = twb + TwSep<NX, 1, R>::WORDS;
+  const long vec = blockIdx.x / nz;
+  const long z = blockIdx.x % nz;
+  const long pbase = z * (long)NX * NX;
+  const int tl = threadIdx.x / G, g = threadIdx.x % G;
+  static_assert(RX == 64, "plane_sep_line maps 64 thread groups");
+  const int ln = plane_sep_line(tl);
+  constexpr int r0b = Pass<NX, 1>::radix(0);
+  constexpr int rlb = Pass<NX, 1>::radix(P - 1);
+  constexpr int rlf = Pass<NX, -1>::radix(P - 1);
+  TwSep<NX, 1, R> tcb;
+  TwSep<NX, -1, R> tcf;
+  // the twiddle tables' loads are issued ahead of the plane's, so that their
+  // LDS stores wait for them alone
+  constexpr int TWN = plane_sep_tw_words();
+  static_assert(TWN <= T, "one twiddle entry per thread");
+  cx<R> twv;
+  if (threadIdx.x < TWN) twv = tws[threadIdx.x];
+  // ---- inverse y FFT of the columns (pass C)
+  {
+    cx<R> w[NR][E];
+#pragma unroll
+    for (int cr = 0; cr < NR; cr++) {
+      const cx<R> *src = tb + vec * N + pbase + (long)(cr * RX + ln) * NX;
+#pragma unroll
+      for (int b = 0; b < E / r0b; b++)
+#pragma unroll
+        for (int k = 0; k < r0b; k++) w[cr][b * r0b + k] = src[in_idx<NX, 1, 0>(g, b, k)];
+    }
+    if (threadIdx.x < TWN) twb[threadIdx.x] = twv;  // twf follows twb
+    __syncthreads();
+    tcb.init(g, twb);
+    tcf.init(g, twf);
+#pragma unroll
+    for (int cr = 0; cr < NR; cr++) {
+      cx<R> *line = buf + LX::f(cr * RX + ln);
+      run_line_sep<NX, 1, LY>(w[cr], g, tcb, line);
+      WaveSync::sync();
+      write_out_sep<NX, 1, P - 1, LY>(w[cr], g, line);
+    }
+  }
+  __syncthreads();  // the transpose
+  // ---- rows: inverse x FFT, epilogue, store psi_j, forward x FFT of psi_j
+  double s[4] = {0.0, 0.0, 0.0, 0.0};  // unused: this path has no partial sums
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int row = r * RX + ln;
+    cx<R> *line = buf + LY::f(row);
+    const long rbase = pbase + (long)row * NX;
+    R V[E];
+#pragma unroll
+    for (int b = 0; b < E / rlb; b++)
+#pragma unroll
+      for (int k = 0; k < rlb; k++) V[b * rlb + k] = epi.pot[rbase + out_idx<NX, 1, P - 1>(g, b, k)];
+    cx<R> x[E];
+    read_in_sep<NX, 1, 0, LX>(x, g, line);
+    run_line_sep<NX, 1, LX>(x, g, tcb, line);
+    // psi_{j-1} of the row, all loads ahead of the stores (which may alias
+    // them: loads issued after a store would wait for it)
+    cx<R> p[E];
+#pragma unroll
+    for (int b = 0; b < E / rlb; b++)
+#pragma unroll
+      for (int k = 0; k < rlb; k++) p[b * rlb + k] = pin.ld(vec, rbase + out_idx<NX, 1, P - 1>(g, b, k));
+#pragma unroll
+    for (int b = 0; b < E / rlb; b++)
+#pragma unroll
+      for (int k = 0; k < rlb; k++) {
+        const long gi = rbase + out_idx<NX, 1, P - 1>(g, b, k);
+        x[b * rlb + k] = epilogue(epi, x[b * rlb + k], p[b * rlb + k], V[b * rlb + k], s);
+        out.st(vec, gi, x[b * rlb + k]);
+      }
+    run_line_sep<NX, -1, LX>(x, g, tcf, line);
+    WaveSync::sync();
+    write_out_sep<NX, -1, P - 1, LX>(x, g, line);
+  }
+  __syncthreads();  // the transpose
+  // ---- forward y FFT of the columns (pass A of step j+1) into plane z of tb
+#pragma unroll
+  for (int cr = 0; cr < NR; cr++) {
+    const int cl = cr * RX + ln;
+    cx<R> *line = buf + LX::f(cl);
+    cx<R> y[E];
+    read_in_sep<NX, -1, 0, LY>(y, g, line);
+    run_line_sep<NX, -1, LY>(y, g, tcf, line);
+    cx<R> *dst = tb + vec * N + pbase + (long)cl * NX;
+#pragma unroll
+    for (int b = 0; b < E / rlf; b++)
+#pragma unroll
+      for (int k = 0; k < rlf; k++) dst[out_idx<NX, -1, P - 1>(g, b, k)] = y[b * rlf + k];
+  }
+}
+
 // Pass C fused with the next pass A, with the plane held in registers and
 // transposed through a buffer of NX/NH columns: k_plane_bwd_w's and
 // k_plane_fwd_w's chunked forms back to back, same arithmetic as
@@ -1138,10 +1320,11 @@ struct HOps {
   static bool rows_bwd_fwd(int nx, cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
                            long nrows, long N, int nvec, const cx<R> *tw, hipStream_t st);
   // pass C fused with the next pass A; false where the plane needs chunks.
-  // chunked: the two-workgroups-per-CU form where plane_fused_h has one
+  // form (SEP_FUSED_*): the kernel variant, where the grid has it; tws: the
+  // separable form's twiddle table (GridDev::twsep32)
   static bool plane_bwd_fwd(int nxy, cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi,
-                            long N, int nz, int nvec, const cx<R> *tw, hipStream_t st,
-                            bool chunked);
+                            long N, int nz, int nvec, const cx<R> *tw, const cx<R> *tws,
+                            hipStream_t st, int form);
 };
 
 #define SEP_LINE_SIZES(X) X(16) X(24) X(32) X(36) X(48) X(64) X(80) X(96) X(128) X(160)
@@ -1319,12 +1502,36 @@ constexpr bool plane_fused_h() {
   return sizeof(R) == 4 && n == 128;
 }
 
+// Grids whose fused kernel has the separable-placement form (k_plane_bwd_fwd_s).
+template <typename R, int n>
+constexpr bool plane_fused_s() {
+  return sizeof(R) == 4 && n == 128;
+}
+
 template <typename R, int n>
 static bool plane_bwd_fwd_launch(cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<R> epi, long N,
-                                 int nz, int nvec, const cx<R> *tw, hipStream_t st, bool chunked) {
+                                 int nz, int nvec, const cx<R> *tw, const cx<R> *tws,
+                                 hipStream_t st, int form) {
   if constexpr (plane_colmajor<n>() && plane_chunks<R, n>() == 1) {
+    if constexpr (plane_fused_s<R, n>()) {
+      if (form == SEP_FUSED_PLANE && !epi.part && tws) {
+        constexpr int T = 1024;
+        constexpr size_t lds = plane_sep_lds();
+        static_assert(lds <= 160 * 1024, "plane buffer exceeds the LDS of a gfx950 workgroup");
+        static bool attr = false;
+        if (!attr) {
+          hipFuncSetAttribute((const void *)k_plane_bwd_fwd_s<T>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+          attr = true;
+        }
+        if (nvec == 0) return true;  // attribute only (before graph capture)
+        hipLaunchKernelGGL((k_plane_bwd_fwd_s<T>), dim3((unsigned)(nz * nvec)), dim3(T), lds, st,
+                           tb, pin, out, epi, N, nz, tws);
+        return true;
+      }
+    }
     if constexpr (plane_fused_h<R, n>()) {
-      if (chunked && !epi.part && !pin.c && pin.im && !out.c && out.im) {
+      if (form == SEP_FUSED_CHUNKED && !epi.part && !pin.c && pin.im && !out.c && out.im) {
         constexpr int T = 512, NH = 2;
         constexpr size_t lds = sizeof(cx<R>) * ((size_t)n * PlaneLds<R, n, NH>::LDc + n);
         static_assert(2 * lds <= 160 * 1024, "two chunk buffers must share the LDS of a CU");
@@ -1357,12 +1564,13 @@ static bool plane_bwd_fwd_launch(cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<
   return false;
 }
 
-bool plane_fused_chunked_supported(int n, int prec) {
+bool plane_fused_form_supported(int n, int prec, int form) {
+  if (form == SEP_FUSED_PLANE) return true;
   if (prec == SEP_PREC_FP64) {
     switch (n) {
 #define C(x) \
   case x:    \
-    return plane_fused_h<double, x>();
+    return form == SEP_FUSED_CHUNKED ? plane_fused_h<double, x>() : plane_fused_s<double, x>();
       SEP_PLANE_SIZES_F64(C)
 #undef C
     }
@@ -1371,7 +1579,7 @@ bool plane_fused_chunked_supported(int n, int prec) {
   switch (n) {
 #define C(x) \
   case x:    \
-    return plane_fused_h<float, x>();
+    return form == SEP_FUSED_CHUNKED ? plane_fused_h<float, x>() : plane_fused_s<float, x>();
     SEP_PLANE_SIZES_F32(C)
 #undef C
   }
@@ -1381,11 +1589,11 @@ bool plane_fused_chunked_supported(int n, int prec) {
 template <>
 bool HOps<float>::plane_bwd_fwd(int nxy, cx<float> *tb, VecView<float> pin, VecView<float> out,
                                 Epi<float> epi, long N, int nz, int nvec, const cx<float> *tw,
-                                hipStream_t st, bool chunked) {
+                                const cx<float> *tws, hipStream_t st, int form) {
   switch (nxy) {
 #define C(n) \
   case n:    \
-    return plane_bwd_fwd_launch<float, n>(tb, pin, out, epi, N, nz, nvec, tw, st, chunked);
+    return plane_bwd_fwd_launch<float, n>(tb, pin, out, epi, N, nz, nvec, tw, tws, st, form);
     SEP_PLANE_SIZES_F32(C)
 #undef C
   }
@@ -1394,11 +1602,12 @@ bool HOps<float>::plane_bwd_fwd(int nxy, cx<float> *tb, VecView<float> pin, VecV
 template <>
 bool HOps<double>::plane_bwd_fwd(int nxy, cx<double> *tb, VecView<double> pin,
                                  VecView<double> out, Epi<double> epi, long N, int nz, int nvec,
-                                 const cx<double> *tw, hipStream_t st, bool chunked) {
+                                 const cx<double> *tw, const cx<double> *tws, hipStream_t st,
+                                 int form) {
   switch (nxy) {
 #define C(n) \
   case n:    \
-    return plane_bwd_fwd_launch<double, n>(tb, pin, out, epi, N, nz, nvec, tw, st, chunked);
+    return plane_bwd_fwd_launch<double, n>(tb, pin, out, epi, N, nz, nvec, tw, tws, st, form);
     SEP_PLANE_SIZES_F64(C)
 #undef C
   }
@@ -1558,6 +1767,11 @@ int hpsi_apply(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, c
 // start of the next step and the x rows of pass C and the next pass A run as
 // one kernel.  Both planes of the vectors must exist (the fused kernels keep
 // the imaginary plane that a real-only view would drop).
+template <typename R>
+static const cx<R> *sep_tws(const GridDev &gd) {
+  return sizeof(R) == 4 ? reinterpret_cast<const cx<R> *>(gd.twsep32) : nullptr;
+}
+
 static bool split_plane(const GridDev &gd, int prec) {
   return !gd.force_lines[prec] && plane_supported(gd.nx, gd.ny, prec);
 }
@@ -1569,8 +1783,8 @@ bool hpsi_fused_supported(const GridDev &gd) {
   if (!split_plane(gd, prec)) return hop_supported(gd.nx) && hop_supported(gd.ny);
   VecView<R> none{nullptr, nullptr, nullptr, 0};
   Epi<R> epi{nullptr, 0.0, 0.0, nullptr};
-  return HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, nullptr, nullptr,
-                                false);
+  return HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, nullptr,
+                                sep_tws<R>(gd), nullptr, gd.fused_form[prec]);
 }
 
 template <typename R>
@@ -1616,8 +1830,8 @@ int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx
   Epi<R> epi{pot, alpha, beta, nullptr};
   bool ok;
   if (plane)
-    ok = fwd_next ? HOps<R>::plane_bwd_fwd(nx, tb, in, out, epi, N, nz, nvec, twx, st,
-                                           gd.fused_chunked[prec])
+    ok = fwd_next ? HOps<R>::plane_bwd_fwd(nx, tb, in, out, epi, N, nz, nvec, twx, sep_tws<R>(gd),
+                                           st, gd.fused_form[prec])
                   : HOps<R>::plane_bwd(nx, tb, in, out, epi, N, nz, nvec, twx, st, nullptr);
   else
     ok = fwd_next
@@ -1650,8 +1864,9 @@ int hpsi_prepare(const GridDev &gd) {
   Epi<R> epi{nullptr, 0.0, 0.0, nullptr};
   HOps<R>::plane_fwd(gd.nx, none, nullptr, gd.N, gd.nz, 0, tw, nullptr);
   HOps<R>::plane_bwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr, nullptr);
-  HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr, false);
-  HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, nullptr, true);
+  for (int form : {SEP_FUSED_PLANE, SEP_FUSED_CHUNKED, SEP_FUSED_SWIZZLED})
+    HOps<R>::plane_bwd_fwd(gd.nx, nullptr, none, none, epi, gd.N, gd.nz, 0, tw, sep_tws<R>(gd),
+                           nullptr, form);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 template int hpsi_prepare<double>(const GridDev &);

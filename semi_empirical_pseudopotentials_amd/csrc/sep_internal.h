@@ -16,6 +16,7 @@ struct GridDev {
   long N;
   void *tw64[3];  // exp(-2 pi i m/n) per axis, complex128
   void *tw32[3];  // same, complex64
+  void *twsep32;  // per-thread-index twiddles of the separable fused kernel (null if none)
   double *kxx, *kyy, *kzz;  // filter/init.c:125-135 axis tables (1/N folded in)
   double ekin;              // Ekinmax / N (filter/init.c:137)
   // pden > 0 turns pass B into the kinetic preconditioner: the spectrum is
@@ -28,9 +29,8 @@ struct GridDev {
   // the filter loop runs pass C and the next pass A as separate kernels
   // (test hook SEP_SEPARATE_PASSES) instead of the fused plane kernel
   bool no_fuse;
-  // per precision: the fused plane kernel takes its chunked form (half a
-  // plane of LDS, two workgroups per CU) where one exists
-  bool fused_chunked[2];
+  // per precision: the form of the fused plane kernel (SEP_FUSED_*)
+  int fused_form[2];
 };
 
 // Spin-orbit neighbourhood lists on the device (sep_so_setup,
@@ -68,7 +68,18 @@ int axpby_dev(double a, const double *x, double b, double *y, long n, hipStream_
 
 bool hop_supported(int n);
 bool plane_supported(int nx, int ny, int prec);
-bool plane_fused_chunked_supported(int n, int prec);
+// Forms of the fused plane kernel (pass C + next pass A):
+//   SEP_FUSED_PLANE     the fastest whole-plane form the grid has (the
+//                       separable placement at complex64 128^2, else the
+//                       swizzled one)
+//   SEP_FUSED_CHUNKED   half a plane of LDS, two workgroups per CU
+//   SEP_FUSED_SWIZZLED  the whole-plane kernel on the swizzled placement
+enum { SEP_FUSED_PLANE = 0, SEP_FUSED_CHUNKED = 1, SEP_FUSED_SWIZZLED = 2 };
+bool plane_fused_form_supported(int n, int prec, int form);
+// Bytes of the separable fused kernel's twiddle table for planes of n (0 if
+// that kernel does not exist for n / prec), and its host construction.
+long plane_sep_tw_bytes(int n, int prec);
+void plane_sep_tw_host(void *tab);
 
 template <typename R>
 int hpsi_prepare(const GridDev &gd);
