@@ -214,9 +214,10 @@ def batched_uneven(sep, decks, read_deck, np, torch, steps=1):
         ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
                             expot=d["expot"])
         tune_ms, fam, lanes = ctx.autotune_ex(sep.SEP_PREC_FP32, (ns + 1) // 2)
-        tune = {"family": ["plane", "plane_chunked", "lines"][fam], "lanes": lanes,
+        names = ("plane", "plane_chunked", "lines", "plane_swizzled")  # SEP_HPSI_* order
+        tune = {"family": names[fam], "lanes": lanes,
                 "ms_per_step": {f: round(float(t), 4) for f, t in
-                                zip(("plane", "plane_chunked", "lines"), tune_ms) if t > 0}}
+                                zip(names, tune_ms) if t > 0}}
         dE, _, _ = ctx.energy_range()
         zn, _ = sep.leja_points(nc)
         an = sep.newton_coeffs(zn, np.linspace(d["Elmin"], d["Elmax"], ms), dE, ctx.vmin,
