@@ -354,13 +354,15 @@ int sep_ortho_fp32(sep_ctx *ctx, long m, double cutoff, double *S, long *kept);
  * enlarged basis holds fewer than m directions. */
 int sep_refine(sep_ctx *ctx, long m, double *eval, double *sige);
 /* Further correction passes (as sep_refine's) on the Ritz pairs of the
- * first m resident pairs (eval ascending) whose energies lie in [lo, hi]:
- * those kw pairs move to the front of the state block and take `passes`
- * passes of their own, each keeping the pairs inside the window first; eval
- * and sige get their energies and variances (zeros beyond kw), and the
- * resident states beyond kw are left unspecified.  Converges the band-edge
- * levels of the FP32 pipeline at a cost in the window's size, not m's.  No
- * counterpart in the reference. */
+ * first m resident pairs (eval ascending, sige their variances) whose
+ * energies lie in [lo, hi]: those kw pairs take `passes` passes of their own
+ * (each Rayleigh-Ritz on [psi, P r, psi_prev] keeping the pairs inside the
+ * window first) and are written back in place, with their eval and sige;
+ * the other pairs are left as they were, and all m pairs end ascending in
+ * energy (reordered only if a refined level crossed a neighbour outside the
+ * window).  Needs room for m + 4 kw resident states.  Converges the levels
+ * of a window of the FP32 pipeline at a cost in the window's size, not m's.
+ * No counterpart in the reference. */
 int sep_refine_window(sep_ctx *ctx, long m, double *eval, double *sige, double lo, double hi,
                       int passes, long *kw);
 

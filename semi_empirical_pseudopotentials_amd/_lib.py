@@ -590,16 +590,20 @@ class Context:
         self._chk(rc, "sep_hf_adjoint")
         return dR, ds, dtab
 
-    def refine_window(self, ev, sg, lo, hi, passes=2):
+    def refine_window(self, ev, sg, lo, hi, passes=2, full=False):
         """sep_refine_window: more correction passes on the resident Ritz pairs
-        with energies in [lo, hi]; returns their (ev, sigma), now the first
-        len(ev) resident states."""
+        with energies in [lo, hi], refined in place.  Returns the (ev, sigma)
+        of the pairs in [lo, hi] after the passes, or with full=True those of
+        all len(ev) resident pairs (ascending)."""
         ev = np.array(ev, dtype=np.float64)
         sg = np.array(sg, dtype=np.float64)
         kw = ctypes.c_long(0)
         self._chk(lib().sep_refine_window(self._h, ev.size, _p(ev), _p(sg), lo, hi, passes,
                                           ctypes.byref(kw)), "sep_refine_window")
-        return ev[:kw.value].copy(), sg[:kw.value].copy()
+        if full:
+            return ev, sg
+        inw = (ev >= lo) & (ev <= hi)
+        return ev[inw].copy(), sg[inw].copy()
 
     def ortho_fp32(self, m, cutoff=1.0e-10):
         """ortho of the FP32 pipeline (sep_ortho_fp32): FP64 Gram eigenbasis with the

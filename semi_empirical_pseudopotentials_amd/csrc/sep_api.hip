@@ -1600,12 +1600,13 @@ extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
 // diagonalised, sigma^2 = c^T (HZ HZ^T) c - e^2 -- and only the k kept pairs
 // (inside [lo, hi] first, then by variance) rotated, with their H psi, both
 // normalized as filter/main.c:122.  One H application per pass (on T).
-// On return prev / hprev hold this pass's X and H X.
-static int window_pass(sep_ctx *c, long k, double *ev, double *sg, double lo, double hi,
-                       double *prev, double *hprev, bool have_prev) {
+// On return prev / hprev hold this pass's X and H X.  X and H X are rows
+// base .. base + k of the two state buffers, which hold base + 4k rows.
+static int window_pass(sep_ctx *c, long base, long k, double *ev, double *sg, double lo,
+                       double hi, double *prev, double *hprev, bool have_prev) {
   const long N = c->gd.N;
   const long nz = have_prev ? 3 * k : 2 * k;
-  double *Z = c->states, *HZ = c->states2;
+  double *Z = c->states + base * N, *HZ = c->states2 + base * N;
   // T = P (H X - e X), normalized, into Z rows k..2k; H T into HZ rows k..2k
   RCHK(ensure_scr(c, k));
   HIPCHK(c, hipMemcpyAsync(Z + k * N, HZ, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
@@ -1734,36 +1735,59 @@ extern "C" int sep_refine_window(sep_ctx *c, long m, double *eval, double *sige,
     return SEP_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
   const long N = c->gd.N;
-  // the window's pairs to the front, in order (idx[j] >= j: in place)
-  std::vector<long> idx;
+  std::vector<long> idx;  // the window's pairs, ascending
   for (long i = 0; i < m; i++)
     if (eval[i] >= lo && eval[i] <= hi) idx.push_back(i);
   const long k = (long)idx.size();
   *kw = k;
   if (k == 0) return SEP_OK;
-  // Z (3k rows) and a k-row output block after it, in both state buffers
-  RCHK(grow_states(c, 4 * k, 1, m));
-  RCHK(grow_states(c, 4 * k, 2, 0));
+  // the passes work on a copy of the window's pairs in rows m .. m + 4k of
+  // both state buffers (X, P R, X_prev and an output block), so the other
+  // resident pairs stay as they are
+  RCHK(grow_states(c, m + 4 * k, 1, m));
+  RCHK(grow_states(c, m + 4 * k, 2, 0));
+  double *X = c->states + m * N;
   std::vector<double> ev(k), sg(k);
   for (long j = 0; j < k; j++) {
     ev[j] = eval[idx[j]];
-    if (idx[j] != j)
-      HIPCHK(c, hipMemcpyAsync(c->states + j * N, c->states + idx[j] * N, sizeof(double) * N,
-                               hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(X + j * N, c->states + idx[j] * N, sizeof(double) * N,
+                             hipMemcpyDeviceToDevice, c->st));
   }
-  RCHK(apply_states(c, c->states, c->states2, k, nullptr, nullptr));  // H X
+  RCHK(apply_states(c, X, c->states2 + m * N, k, nullptr, nullptr));  // H X
   double *prev = nullptr;
   HIPCHK(c, hipMalloc(&prev, sizeof(double) * 2 * k * N));
   int rc = SEP_OK;
   for (int p = 0; p < passes && rc == SEP_OK; p++)
-    rc = window_pass(c, k, ev.data(), sg.data(), lo, hi, prev, prev + k * N, p > 0);
+    rc = window_pass(c, m, k, ev.data(), sg.data(), lo, hi, prev, prev + k * N, p > 0);
   hipFree(prev);
   RCHK(rc);
-  for (long j = 0; j < m; j++) eval[j] = sige[j] = 0.0;
+  // the refined pairs (ascending) back into the window's positions
   for (long j = 0; j < k; j++) {
-    eval[j] = ev[j];
-    sige[j] = sg[j];
+    HIPCHK(c, hipMemcpyAsync(c->states + idx[j] * N, X + j * N, sizeof(double) * N,
+                             hipMemcpyDeviceToDevice, c->st));
+    eval[idx[j]] = ev[j];
+    sige[idx[j]] = sg[j];
   }
+  // a refined level that crossed a neighbour outside the window: all pairs
+  // back in ascending order (states staged in the second buffer)
+  if (!std::is_sorted(eval, eval + m)) {
+    std::vector<long> ord(m);
+    for (long i = 0; i < m; i++) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](long a, long b) { return eval[a] < eval[b]; });
+    for (long i = 0; i < m; i++)
+      HIPCHK(c, hipMemcpyAsync(c->states2 + i * N, c->states + ord[i] * N, sizeof(double) * N,
+                               hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->states, c->states2, sizeof(double) * m * N,
+                             hipMemcpyDeviceToDevice, c->st));
+    std::vector<double> e2(m), s2(m);
+    for (long i = 0; i < m; i++) {
+      e2[i] = eval[ord[i]];
+      s2[i] = sige[ord[i]];
+    }
+    std::copy(e2.begin(), e2.end(), eval);
+    std::copy(s2.begin(), s2.end(), sige);
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return SEP_OK;
 }
 

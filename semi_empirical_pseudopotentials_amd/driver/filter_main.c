@@ -11,7 +11,10 @@
  *   --dir D            deck / output directory (default .)
  *   --seed S           ran_nrc seed (overrides seed.par; 0 = clock as main.c:79)
  *   --precision P      fp64 (default) or fp32: complex64 filter, FP64-Gram ortho,
- *                      FP64 Rayleigh-Ritz and one FP64 preconditioned correction pass
+ *                      FP64 Rayleigh-Ritz and sigma, then FP64 preconditioned
+ *                      correction passes (sep_refine_window) on the pairs with
+ *                      energies in [Elmin, Elmax] of input.par
+ *   --refine-passes P  number of those passes (default 10; 0 = none)
  *   --device N         HIP device ordinal
  *   --lanes L          filter lanes (groups of random states on concurrent
  *                      streams): 0 = automatic (default), 1 = one stream, up
@@ -187,6 +190,7 @@ static void write_hmat_log(long mstot, long ngrid) {
 int main(int argc, char **argv) {
   long seed_opt = 0;
   int have_seed_opt = 0, prec = SEP_PREC_FP64, device = 0, psi_filt = 0, resume = 0, lanes = 0;
+  int refine_passes = 10;
   for (int i = 1; i < argc; i++) {
     if (!strcmp(argv[i], "--dir") && i + 1 < argc)
       g_dir = argv[++i];
@@ -206,12 +210,26 @@ int main(int argc, char **argv) {
     else if (!strcmp(argv[i], "--lanes") && i + 1 < argc) {
       char *end = NULL;
       const long v = strtol(argv[++i], &end, 10);
-      if (!end || *end || v < 0 || v > SEP_MAX_LANES) die("--lanes must be an integer in 0..4");
+      if (!end || *end || v < 0 || v > SEP_MAX_LANES) {
+        char msg[64];
+        snprintf(msg, sizeof msg, "--lanes must be an integer in 0..%d", SEP_MAX_LANES);
+        die(msg);
+      }
       lanes = (int)v;
+    } else if (!strcmp(argv[i], "--refine-passes") && i + 1 < argc) {
+      char *end = NULL;
+      const long v = strtol(argv[++i], &end, 10);
+      if (!end || *end || v < 0 || v > 1000) die("--refine-passes must be an integer in 0..1000");
+      refine_passes = (int)v;
     } else if (!strcmp(argv[i], "--psi-filt"))
       psi_filt = 1;
     else if (!strcmp(argv[i], "--resume-filt"))
       resume = 1;
+    else {
+      char msg[300];
+      snprintf(msg, sizeof msg, "unknown option or missing value: %.250s", argv[i]);
+      die(msg);
+    }
   }
   stamp();
   separation();
@@ -229,6 +247,8 @@ int main(int argc, char **argv) {
   printf("Kinetic energy maximum = % .2f\n", 10.0);
   printf("Number of openMP threads used = %ld\n", d.nthreads);
   printf("Precision: %s\n", prec == SEP_PREC_FP64 ? "fp64" : "fp32 filter/ortho + fp64 Rayleigh-Ritz and refinement");
+  if (prec == SEP_PREC_FP32)
+    printf("FP64 correction passes on the pairs in [Elmin, Elmax]: %d\n", refine_passes);
   if (d.has_expot)
     printf("expot.par was detected: e = % .6f x0 = % .6f kT = % .6f\n", d.expot[0], d.expot[1],
            d.expot[2]);
@@ -418,7 +438,16 @@ int main(int argc, char **argv) {
   double *sige = (double *)malloc(sizeof(double) * kept);
   CHECK(sep_rayleigh_ritz(ctx, kept, eval));
   write_hmat_log(kept, N);
-  if (prec == SEP_PREC_FP32) CHECK(sep_refine(ctx, kept, eval, sige));
+  if (prec == SEP_PREC_FP32) {
+    /* the Ritz pairs' variances, then the window's pairs refined in place */
+    CHECK(sep_sigma(ctx, kept, sige));
+    if (refine_passes > 0) {
+      long kw = 0;
+      CHECK(sep_refine_window(ctx, kept, eval, sige, d.Elmin, d.Elmax, refine_passes, &kw));
+      printf("refined %ld Ritz pairs in [% .4f, % .4f] with %d FP64 correction passes\n", kw,
+             d.Elmin, d.Elmax, refine_passes);
+    }
+  }
   printf("done calculating Hmat, CPU time (sec) %g, wall run time (sec) %g\n",
          ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
   fflush(stdout);
@@ -434,7 +463,7 @@ int main(int argc, char **argv) {
     free(psi);
   }
   /* sigma and eval.dat (main.c:135-142) */
-  if (prec != SEP_PREC_FP32) CHECK(sep_sigma(ctx, kept, sige));  /* refine computed it */
+  if (prec != SEP_PREC_FP32) CHECK(sep_sigma(ctx, kept, sige));  /* fp32: computed above */
   {
     FILE *pf = out_file("eval.dat", "w");
     if (!pf) die("cannot write eval.dat");

@@ -80,9 +80,10 @@ def edge_window(ev, sg, band=0.5 * EV, below=0.1 * EV, above=0.1 * EV, sig_edge=
 def fp32_pipeline(ctx, m, passes=10, band=0.5 * EV):
     """FP32 ortho + FP64 Rayleigh-Ritz + sigma on the m resident filtered
     states of a complex64 filter, then `passes` FP64 correction passes on the
-    pairs of the band-edge window.  Returns a dict: kept, ev / sg of all
-    Ritz pairs before the passes, window (lo, hi, e0), ev_w / sg_w of the
-    refined window pairs (now the first resident states)."""
+    pairs of the band-edge window (refined in place among the resident
+    pairs).  Returns a dict: kept, ev / sg of all Ritz pairs before the
+    passes, window (lo, hi, e0), ev_w / sg_w of the refined pairs in the
+    window."""
     kept, _ = ctx.ortho_fp32(m)
     ev = ctx.rayleigh_ritz(kept)
     sg = ctx.sigma(kept)

@@ -125,3 +125,19 @@ def test_filter_driver_rejects_bad_lanes(tmp_path):
                            capture_output=True, text=True, timeout=60)
         assert r.returncode == 1
         assert "--lanes must be" in open(tmp_path / "error").read()
+
+
+def test_filter_driver_rejects_unknown_options(tmp_path):
+    """bin/filter refuses an unknown option (a misspelt --precision would
+    otherwise run FP64 unnoticed) and a bad --refine-passes, before reading
+    the deck or touching a device."""
+    import subprocess
+    from semi_empirical_pseudopotentials_amd._lib import FILTER_BIN
+    for args, msg in ((["--precison", "fp32"], "unknown option or missing value: --precison"),
+                      (["--refine-passes", "x"], "--refine-passes must be"),
+                      (["--refine-passes", "-2"], "--refine-passes must be"),
+                      (["--seed"], "unknown option or missing value: --seed")):
+        r = subprocess.run([FILTER_BIN, "--dir", str(tmp_path), *args], capture_output=True,
+                           text=True, timeout=60)
+        assert r.returncode == 1, args
+        assert msg in open(tmp_path / "error").read(), args

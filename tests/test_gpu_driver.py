@@ -89,6 +89,32 @@ def test_driver_in13_reference(sep, tmp_path):
     assert np.abs(pairs[:, 0] - pairs[:, 1]).max() < 1e-8
 
 
+def test_driver_in13_fp32_window_vs_reference(sep, tmp_path):
+    """--precision fp32 on In13_P16: complex64 filter and ortho, FP64
+    Rayleigh-Ritz and sigma, then 10 FP64 correction passes on the pairs in
+    [Elmin, Elmax] of input.par (-0.23, -0.05 Ha).  Every level of the
+    reference's own eval.dat (FP64) converged there (sigma < 1e-4) is
+    matched one to one, multiplet by multiplet, within 1 meV."""
+    from tests.deckutil import compare_levels
+    print("case driver In13_P16 fp32", flush=True)
+    out = str(tmp_path / "in13_32")
+    log = _run_filter(os.path.join(GOLDEN, "In13_P16"), out, "--seed", "-1131359211",
+                      "--precision", "fp32")
+    assert "FP64 correction passes" in log
+    ref_e, ref_s = read_eval(os.path.join(GOLDEN, "In13_P16", "eval.dat"))
+    got_e, got_s = read_eval(os.path.join(out, "eval.dat"))
+    assert np.all(np.diff(got_e) >= 0)
+    N = 36 ** 3
+    assert os.path.getsize(os.path.join(out, "psi.dat")) == 8 * N * got_e.size
+    mev = 1e-3 / 27.211386245988
+    nc, nl, d = compare_levels(ref_e, ref_s, got_e, got_s, -0.23, -0.05, mev)
+    print(f"fp32 driver vs reference eval.dat in [-0.23, -0.05]: {nl} levels in {nc} "
+          f"multiplets, max |dE| {d / mev:.4f} meV", flush=True)
+    # every converged reference level of the window: 7 (3 multiplets)
+    inwin = (ref_s < 1e-4) & (ref_e >= -0.23) & (ref_e <= -0.05)
+    assert nl == inwin.sum() == 7
+
+
 def test_driver_resume_from_filtered_states(sep, small_deck, tmp_path):
     """--psi-filt writes the filtered states; --resume-filt in a copy of that
     directory skips the filter and reproduces eval.dat and psi.dat exactly."""

@@ -51,6 +51,16 @@ def run(sep, tmp_path_factory):
     sg_rr = ctx.sigma(k32)
     ev32, sg32 = ctx.refine(ev_rr)
     out.update(k32=k32, S32=S32, Q32=Q32, ev_rr=ev_rr, sg_rr=sg_rr, ev32=ev32, sg32=sg32)
+    # window passes in place on the refined pairs: window edges in the widest
+    # gaps near a third and a half of the spectrum
+    print("case refine window in place", flush=True)
+    gaps = np.diff(ev32)
+    i = k32 // 3 + int(np.argmax(gaps[k32 // 3:k32 // 3 + 8]))
+    j = k32 // 2 + int(np.argmax(gaps[k32 // 2:k32 // 2 + 8]))
+    lo, hi = 0.5 * (ev32[i] + ev32[i + 1]), 0.5 * (ev32[j] + ev32[j + 1])
+    before = ctx.states_download(k32)
+    evw, sgw = ctx.refine_window(ev32, sg32, lo, hi, passes=3, full=True)
+    out.update(win=(lo, hi), before=before, after=ctx.states_download(k32), evw=evw, sgw=sgw)
     # the double-double FP64 ortho of the same complex64-filtered block
     ctx.states_upload(blk32)
     out["S64_same"] = ctx.ortho(ns * ms)[1]
@@ -80,6 +90,24 @@ def test_refine_lowers_variance(run):
     sg_rr, sg32 = run["sg_rr"], run["sg32"]
     assert np.all(np.isfinite(sg32))
     assert np.median(sg32) <= np.median(sg_rr) * (1 + 1e-9)
+
+
+def test_refine_window_in_place(run):
+    """sep_refine_window refines the pairs in [lo, hi] in place: the other
+    resident pairs (states, energies, variances) are untouched, the window's
+    variances drop, and the pairs stay ascending in energy."""
+    lo, hi = run["win"]
+    ev, sg, evw, sgw = run["ev32"], run["sg32"], run["evw"], run["sgw"]
+    inw = (ev >= lo) & (ev <= hi)
+    assert inw.sum() >= 3 and (~inw).sum() >= 3
+    assert np.all(np.diff(evw) >= 0)
+    assert np.array_equal(evw[~inw], ev[~inw]) and np.array_equal(sgw[~inw], sg[~inw])
+    b, a = run["before"], run["after"]
+    assert np.array_equal(a[~inw], b[~inw])
+    assert not np.array_equal(a[inw], b[inw])
+    print("window pairs", inw.sum(), "median sigma", np.median(sg[inw]), "->",
+          np.median(sgw[inw]), flush=True)
+    assert np.median(sgw[inw]) < np.median(sg[inw])
 
 
 def test_fp32_pipeline_edge_levels_within_1mev(run):
