@@ -42,6 +42,9 @@ struct sep_ctx {
   size_t gwork_bytes = 0;
   void *ring = nullptr;
   size_t ring_bytes = 0;
+  // a filter pass is using the ring (outside one the ring holds nothing
+  // live: the filtered states are in the state block)
+  bool ring_busy = false;
   // live kernel timing of the filter loop (sep_set_timing): one sampled
   // Newton step per ring block (passes A, B, C) and that block's accumulation
   bool timing = false;
@@ -91,6 +94,31 @@ static int fail(sep_ctx *c, int code, const char *fmt, ...) {
     if (r_ != SEP_OK) return r_; \
   } while (0)
 
+// hipMalloc that, when the memory is not there, frees the filter's ring if
+// no filter pass is using it (it is sized from the free memory, up to 60 %
+// of it, and kept between filters) and tries once more.
+static hipError_t dev_malloc(sep_ctx *c, void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess || !c->ring || c->ring_busy) return e;
+  (void)hipGetLastError();  // the failed allocation must not be reported by a later call
+  hipStreamSynchronize(c->st);
+  hipFree(c->ring);
+  c->ring = nullptr;
+  c->ring_bytes = 0;
+  return hipMalloc(p, bytes);
+}
+// Marks the contexts' rings as in use for a scope (filter passes).
+struct RingBusy {
+  std::vector<sep_ctx *> cs;
+  void add(sep_ctx *c) {
+    c->ring_busy = true;
+    cs.push_back(c);
+  }
+  ~RingBusy() {
+    for (sep_ctx *c : cs) c->ring_busy = false;
+  }
+};
+
 static int ensure(sep_ctx *c, void **p, size_t *cap, size_t bytes) {
   if (*cap >= bytes && *p) return SEP_OK;
   if (*p) {  // work queued on the old buffer finishes before it is freed
@@ -100,7 +128,7 @@ static int ensure(sep_ctx *c, void **p, size_t *cap, size_t bytes) {
   *p = nullptr;
   *cap = 0;
   if (bytes == 0) return SEP_OK;
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = dev_malloc(c, p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
     return fail(c, SEP_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
@@ -124,7 +152,7 @@ static int grow_states(sep_ctx *c, long m, int which, long keep) {
   double *&p = which == 1 ? c->states : c->states2;
   if (p && cap >= m) return SEP_OK;
   double *q = nullptr;
-  HIPCHK(c, hipMalloc(&q, sizeof(double) * (size_t)m * c->gd.N));
+  HIPCHK(c, dev_malloc(c, (void **)&q, sizeof(double) * (size_t)m * c->gd.N));
   if (p && keep > 0)
     HIPCHK(c, hipMemcpyAsync(q, p, sizeof(double) * (size_t)(keep < cap ? keep : cap) * c->gd.N,
                              hipMemcpyDeviceToDevice, c->st));
@@ -156,7 +184,7 @@ struct DevBuf {
   }
 };
 static int dev_alloc(sep_ctx *c, DevBuf &b, size_t bytes) {
-  HIPCHK(c, hipMalloc(&b.p, bytes > 0 ? bytes : 8));
+  HIPCHK(c, dev_malloc(c, &b.p, bytes > 0 ? bytes : 8));
   return SEP_OK;
 }
 static int dev_upload(sep_ctx *c, DevBuf &b, const void *src, size_t bytes) {
@@ -940,6 +968,8 @@ template <typename R>
 static int filter_impl(sep_ctx *c, const double *psi0, bool psi0_on_device, long ns,
                        const double *an, const double *zn, long nc, long ms, double vmin,
                        double dE, bool so = false) {
+  RingBusy busy;
+  busy.add(c);
   FilterRun<R> f;
   int rc = filter_start<R>(f, c, psi0, psi0_on_device, ns, an, zn, nc, ms, vmin, dE, so);
   const int L = rc == SEP_OK ? filter_lanes<R>(c, f) : 1;
@@ -979,6 +1009,8 @@ static int filter_batch_impl(int nd, sep_ctx *const *ctxs, const void *const *ps
                              const long *ns, const double *const *an, const double *const *zn,
                              const long *nc, const long *ms, const double *vmin, const double *dE,
                              int mode) {
+  RingBusy busy;
+  for (int d = 0; d < nd; d++) busy.add(ctxs[d]);
   std::vector<FilterRun<R>> runs(nd);
   std::vector<hipGraphExec_t> execs(nd, nullptr);
   std::vector<hipGraph_t> graphs(nd, nullptr);
@@ -1755,7 +1787,7 @@ extern "C" int sep_refine_window(sep_ctx *c, long m, double *eval, double *sige,
   }
   RCHK(apply_states(c, X, c->states2 + m * N, k, nullptr, nullptr));  // H X
   double *prev = nullptr;
-  HIPCHK(c, hipMalloc(&prev, sizeof(double) * 2 * k * N));
+  HIPCHK(c, dev_malloc(c, (void **)&prev, sizeof(double) * 2 * k * N));
   int rc = SEP_OK;
   for (int p = 0; p < passes && rc == SEP_OK; p++)
     rc = window_pass(c, m, k, ev.data(), sg.data(), lo, hi, prev, prev + k * N, p > 0);
@@ -2035,6 +2067,8 @@ extern "C" int sep_bench_hpsi(sep_ctx *c, int precision, long nvec, long ms, lon
   if (!c || nvec < 1 || ms < 1 || iters < 1 || !kernel_ms) return SEP_ERR_ARG;
   if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
   HIPCHK(c, hipSetDevice(c->device));
+  RingBusy busy;  // the benchmark works in the ring
+  busy.add(c);
   if (precision == SEP_PREC_FP64) return bench_impl<double>(c, nvec, ms, iters, kernel_ms);
   if (precision == SEP_PREC_FP32) return bench_impl<float>(c, nvec, ms, iters, kernel_ms);
   return fail(c, SEP_ERR_ARG, "unknown precision %d", precision);
@@ -2610,7 +2644,7 @@ int sep_ortho_spinor(sep_ctx *c, long m, double cutoff, double *S, long *kept) {
     RCHK(ensure_states(c, 8 * k, 2));
     RCHK(stack_x_ix(c, k));
     double *G_d = nullptr;
-    HIPCHK(c, hipMalloc(&G_d, sizeof(double) * 2 * k * k));
+    HIPCHK(c, dev_malloc(c, (void **)&G_d, sizeof(double) * 2 * k * k));
     int rc = sub_gram_k(c, c->states, n4, k, c->states, n4, k, n4, false, G_d);
     if (rc == SEP_OK)
       rc = sub_gram_k(c, c->states2 + k * n4, n4, k, c->states, n4, k, n4, false, G_d + k * k);
@@ -2655,7 +2689,7 @@ int sep_rayleigh_ritz_spinor(sep_ctx *c, long m, double *eval) {
   double *HX = c->states2 + 2 * m * n4;
   RCHK(apply_spinors(c, c->states, HX, m));
   double *P_d = nullptr;
-  HIPCHK(c, hipMalloc(&P_d, sizeof(double) * 2 * m * m));
+  HIPCHK(c, dev_malloc(c, (void **)&P_d, sizeof(double) * 2 * m * m));
   // P(s, t) = <x_s|H x_t> dv (Hmat.c:30-40)
   int rc = sub_gram_k(c, c->states, n4, m, HX, n4, m, n4, false, P_d);
   if (rc == SEP_OK) rc = sub_gram_k(c, c->states2 + m * n4, n4, m, HX, n4, m, n4, false, P_d + m * m);
