@@ -395,6 +395,10 @@ int sep_energy_range_spinor(sep_ctx *ctx, double *vmin, double *dE, long *iters)
  * block, spinor jns * ms + jms as the reference's psitot. */
 int sep_filter_spinor_dev(sep_ctx *ctx, const void *psi0_dev, long nspinor, const double *an,
                           const double *zn, long nc, long ms, double vmin, double dE);
+/* The same on nspinor host spinors (4 N doubles each), for callers without
+ * device memory of their own (the filter_spinors driver). */
+int sep_filter_spinor(sep_ctx *ctx, const double *psi0, long nspinor, const double *an,
+                      const double *zn, long nc, long ms, double vmin, double dE);
 /* energy_all() / calc_sigma_E() (spinors/filterSpinors/energy.c:28-49,
  * 117-147) on the first m resident spinors. */
 int sep_energy_spinor(sep_ctx *ctx, long m, double *ene);

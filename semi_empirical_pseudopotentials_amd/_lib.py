@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libsep.so")
 FILTER_BIN = os.path.join(_HERE, "bin", "filter")
 COUPLINGS_BIN = os.path.join(_HERE, "bin", "couplings")
+FILTER_SPINORS_BIN = os.path.join(_HERE, "bin", "filter_spinors")
 
 SEP_OK = 0
 SEP_INTERP_LINEAR = 0
@@ -109,6 +110,7 @@ SIGNATURES = {
     "sep_apply_h_spinor_dev": (_I, [_V, _V, _V, _L]),
     "sep_energy_range_spinor": (_I, [_V, _dp, _dp, _lp]),
     "sep_filter_spinor_dev": (_I, [_V, _V, _L, _dp, _dp, _L, _L, _D, _D]),
+    "sep_filter_spinor": (_I, [_V, _dp, _L, _dp, _dp, _L, _L, _D, _D]),
     "sep_energy_spinor": (_I, [_V, _L, _dp]),
     "sep_sigma_spinor": (_I, [_V, _L, _dp]),
     "sep_ortho_spinor": (_I, [_V, _L, _D, _dp, _lp]),

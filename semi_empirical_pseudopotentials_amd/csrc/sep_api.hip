@@ -2577,14 +2577,14 @@ int sep_energy_range_spinor(sep_ctx *c, double *vmin, double *dE, long *iters) {
   return SEP_OK;
 }
 
-int sep_filter_spinor_dev(sep_ctx *c, const void *psi0_dev, long nspinor, const double *an,
-                          const double *zn, long nc, long ms, double vmin, double dE) {
-  if (!c || !psi0_dev || nspinor < 1 || !an || !zn || nc < 1 || ms < 1 || !(dE > 0.0))
+static int filter_spinor(sep_ctx *c, const double *psi0, bool on_device, long nspinor,
+                         const double *an, const double *zn, long nc, long ms, double vmin,
+                         double dE) {
+  if (!c || !psi0 || nspinor < 1 || !an || !zn || nc < 1 || ms < 1 || !(dE > 0.0))
     return SEP_ERR_ARG;
   if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
   HIPCHK(c, hipSetDevice(c->device));
-  RCHK(filter_impl<double>(c, static_cast<const double *>(psi0_dev), true, 4 * nspinor, an, zn, nc,
-                           ms, vmin, dE, true));
+  RCHK(filter_impl<double>(c, psi0, on_device, 4 * nspinor, an, zn, nc, ms, vmin, dE, true));
   const long N = c->gd.N, m = nspinor * ms;
   RCHK(ensure_states(c, 4 * m, 2));
   if (spinor_gather_rows_dev(c->states, c->states2, N, nspinor, ms, c->st) != SEP_OK)
@@ -2592,6 +2592,17 @@ int sep_filter_spinor_dev(sep_ctx *c, const void *psi0_dev, long nspinor, const 
   std::swap(c->states, c->states2);
   std::swap(c->cap1, c->cap2);
   return normalize_spinors(c, c->states, m);  // filter.c:29
+}
+
+int sep_filter_spinor_dev(sep_ctx *c, const void *psi0_dev, long nspinor, const double *an,
+                          const double *zn, long nc, long ms, double vmin, double dE) {
+  return filter_spinor(c, static_cast<const double *>(psi0_dev), true, nspinor, an, zn, nc, ms,
+                       vmin, dE);
+}
+
+int sep_filter_spinor(sep_ctx *c, const double *psi0, long nspinor, const double *an,
+                      const double *zn, long nc, long ms, double vmin, double dE) {
+  return filter_spinor(c, psi0, false, nspinor, an, zn, nc, ms, vmin, dE);
 }
 
 int sep_energy_spinor(sep_ctx *c, long m, double *ene) {

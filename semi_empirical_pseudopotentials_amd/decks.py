@@ -121,6 +121,30 @@ def write_deck(path, syms, xyz, nx, ny, nz, ms, ns, nc, elmin, elmax, nthreads=1
     return path
 
 
+def write_spinor_deck(path, syms, xyz, nx, ny, nz, dx, ms, ns, nc, elmin, elmax, pots,
+                      nthreads=1, flagkb=0, seed=None):
+    """A deck of the spin-orbit filter (spinors/filterSpinors): input.par as
+    its init_size reads it (nx ny nz dx ms ns nc Elmin Elmax nthreads flagkb,
+    init.c:5-21), conf.par and pot<Sym>.par (+ seed.par)."""
+    os.makedirs(path, exist_ok=True)
+    with open(os.path.join(path, "input.par"), "w") as f:
+        f.write(f"{nx}\n{ny}\n{nz}\n{dx}\n{ms}\n{ns}\n{nc}\n{elmin!r}\n{elmax!r}\n{nthreads}\n"
+                f"{flagkb}\n")
+    with open(os.path.join(path, "conf.par"), "w") as f:
+        f.write(f"{len(syms)}\n")
+        for s, p in zip(syms, xyz):
+            f.write(f"{s} {p[0]:.8f} {p[1]:.8f} {p[2]:.8f}\n")
+    for sym, prm in pots.items():
+        r, v = pot_table(*prm)
+        with open(os.path.join(path, f"pot{sym}.par"), "w") as f:
+            for ri, vi in zip(r, v):
+                f.write(f"{ri:.4f} {vi:.10e}\n")
+    if seed is not None:
+        with open(os.path.join(path, "seed.par"), "w") as f:
+            f.write(f"{seed}\n")
+    return path
+
+
 def box_from_atoms(xyz, nx, ny, nz):
     """Box and spacing by the reference rule (filter/init.c:90-122)."""
     xyz = np.asarray(xyz, dtype=float)

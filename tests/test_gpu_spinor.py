@@ -4,6 +4,8 @@ nonlocal spin-orbit term to 1e-12, then the pipeline -- energy range,
 filter, ortho, Rayleigh-Ritz, sigma -- on a 16^3 grid with an In/As
 cluster (8 random spinors x 4 targets, 512 Newton steps).  The filtered spinors' energies (eval-filt) agree to 1e-10 Ha and
 the converged Ritz values (sigma < 1e-6) to 1e-9 Ha."""
+import os
+
 import numpy as np
 import pytest
 
@@ -128,3 +130,92 @@ def test_spinor_pipeline_vs_oracle(sep, oracle):
     # sigma of the GPU Ritz spinors agrees where converged
     g = np.array([sg[np.abs(ev - e).argmin()] for e in oev[conv]])
     assert np.all(g < 2e-6)
+
+
+def test_spinor_driver_vs_oracle(sep, oracle, tmp_path):
+    """bin/filter_spinors end to end on an In/As cluster deck (24^3 after the
+    box rule, dx 0.8, Ekinmax 20, 4 random spinors x 4 targets, nc 1024,
+    seed -1001; tables deep enough that the bound levels lie well apart, so
+    the energy range's Vmin falls below them) against the spinor oracle on the same tables, grid, atoms,
+    seed and Newton coefficients: energy range to 1e-9, filtered energies
+    (eval-filt) to 1e-9 Ha, the converged Ritz values (sigma < 1e-6) of
+    eval.dat to 1e-9 Ha, the file formats (psi.dat complex spinors, conf.dat,
+    list.dat)."""
+    import re
+    import subprocess
+    from oracle import spinor as S
+    from semi_empirical_pseudopotentials_amd import decks
+    from semi_empirical_pseudopotentials_amd._lib import FILTER_SPINORS_BIN
+    syms = ["In", "As", "As", "As", "As"]
+    xyz, vso = _cluster()
+    pots = {"In": (-3.0, 1.5, 0.0, 1.0), "As": (-2.5, 1.4, 0.0, 1.0)}
+    n, d, ms, ns, nc = 24, 0.8, 4, 4, 1024
+    grid, origin = (n, n, n), (-n * d / 2,) * 3
+    xc = xyz - xyz.mean(axis=0)
+    # the tables as the driver reads and shifts them (read.c:143-151)
+    path = str(tmp_path / "deck")
+    decks.write_spinor_deck(path, syms, xyz, n, n, n, d, ms, ns, nc, -1.0, 0.0, pots)
+    vr = np.zeros((2, 8192))
+    pt = np.zeros((2, 8192))
+    npot, dr = [], []
+    for t, s in enumerate(("In", "As")):
+        tab = np.loadtxt(f"{path}/pot{s}.par")
+        vr[t, :len(tab)], pt[t, :len(tab)] = tab[:, 0], tab[:, 1] - tab[-1, 1]
+        npot.append(len(tab))
+        dr.append(tab[1, 0] - tab[0, 0])
+    types = np.array([0, 1, 1, 1, 1])
+    print("case spinor driver: oracle V and energy range", flush=True)
+    V = oracle.build_potential(grid, origin, (d, d, d), xc, types, np.ones(5), npot, vr, pt, dr)
+    lists = S.so_lists(grid, origin, (d, d, d), xc, vso)
+    ksq = oracle.ksqr(n, n, n, d, d, d, 20.0)
+    dv = d ** 3
+    ovmin, odE, _ = S.energy_range(grid, ksq, V, lists, dv, d)
+    e0 = ovmin / 1.2 if ovmin < 0 else ovmin / 0.8  # Emin, from Vmin = Emin - 0.2 |Emin|
+    elmin, elmax = e0, e0 + 0.4
+    decks.write_spinor_deck(path, syms, xyz, n, n, n, d, ms, ns, nc, elmin, elmax, pots)
+    print("case spinor driver: run", flush=True)
+    r = subprocess.run([FILTER_SPINORS_BIN, "--dir", path, "--seed", "-1001"], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    m = re.search(r"energy range: Vmin = (\S+) dE = (\S+)", r.stdout)
+    vmin, dE = float(m.group(1)), float(m.group(2))
+    assert "nx = 24  ny = 24  nz = 24" in r.stdout
+    assert abs(vmin - ovmin) < 1e-9 * abs(ovmin) and abs(dE - odE) < 1e-9 * odE
+    print("case spinor driver: oracle filter", flush=True)
+    zn, _ = sep.leja_points(nc)
+    an = sep.newton_coeffs(zn, np.linspace(elmin, elmax, ms), dE, vmin, (nc / (2.5 * dE)) ** 2)
+    rng = oracle.Rng()
+    idum = np.array([-1001], dtype=np.int64)
+    N = n ** 3
+    psi0 = [S.init_spinor(rng, idum, N, dv) for _ in range(ns)]
+    ofilt = np.concatenate([S.filter_spinor(grid, ksq, V, lists, dv, d, an, zn, dE, vmin, p)
+                            for p in psi0])
+    oene = np.array([float(np.sum(np.conj(p) * S.hamiltonian(grid, ksq, V, lists, p, dv, d)).real
+                           * dv) for p in ofilt])
+    gene = np.concatenate([np.loadtxt(f"{path}/eval-filt-0-{j}.dat")[:, 1] for j in range(ns)])
+    print("eval-filt max |dE|", np.abs(gene - oene).max(), flush=True)
+    # measured 1.4e-10: dE = 25 Ha here (Ekinmax 20) and 1024 Newton steps
+    # amplify rounding more than at the pipeline test's 6 Ha
+    assert np.abs(gene - oene).max() < 1e-9
+    U, osv, ok = S.portho(ofilt)
+    U = np.array([u / np.sqrt(S.spinor_norm2(u, dv)) for u in U])
+    oev, opsi = S.hmat(grid, ksq, V, lists, dv, d, U)
+    osg = S.sigma_e(grid, ksq, V, lists, dv, d, opsi)
+    ev = np.loadtxt(f"{path}/eval.dat")
+    kept = ev.shape[0]
+    assert os.path.getsize(f"{path}/psi.dat") == kept * 2 * N * 16
+    conv = osg < 1e-6
+    print("kept", kept, "oracle", ok, "converged", conv.sum(), flush=True)
+    assert conv.sum() >= 6
+    err = np.array([np.abs(ev[:, 1] - e).min() for e in oev[conv]])
+    print("converged Ritz values max |dE| Ha", err.max(), flush=True)
+    assert err.max() < 1e-9
+    # psi.dat holds the Ritz spinors: the lowest one's H expectation is its eval
+    with open(f"{path}/psi.dat", "rb") as f:
+        z = np.frombuffer(f.read(2 * N * 16), dtype=np.complex128).reshape(2, N)
+    e_psi = float(np.sum(np.conj(z) * S.hamiltonian(grid, ksq, V, lists, z, dv, d)).real * dv)
+    assert abs(e_psi - ev[0, 1]) < 1e-8
+    conf = open(f"{path}/conf.dat").read().split()
+    assert conf[0] == "In" and len(conf) == 5 * 6
+    lst = np.loadtxt(f"{path}/list.dat", dtype=np.int64)
+    assert np.array_equal(lst[:, 1], [e["jxyz"].size for e in lists])
