@@ -167,6 +167,76 @@ def precision_check(sep, decks, read_deck, np):
                             res["fp32"][1], conv=1e-3))
 
 
+def physical_check(sep, decks, read_deck, np):
+    """The 1 meV gate at a real band edge: the 4.0 nm CdSe dot at 128^3 on the
+    reference's own tables (potCd.par, potSe.par of its In13_P16 test deck;
+    analytic P1/P2 passivation), window [-0.26, -0.12] Ha around its gap, 16
+    random states x 16 targets, nc 4096.  The FP32 pipeline finds the
+    conduction edge from its own Ritz values (largest gap among pairs with
+    sigma < 1e-2), refines the pairs from 0.1 eV below to 1.1 eV above it
+    (10 FP64 passes, in place), and its levels within 1 eV of the edge are
+    compared one to one with the FP64 pipeline's (sigma < 1e-4)."""
+    from semi_empirical_pseudopotentials_amd import levels
+    import shutil
+    import tempfile
+    ev_ha = 1.0 / 27.211386245988
+    n, ns, ms, nc, lo, hi = 128, 16, 16, 4096, -0.26, -0.12
+    tmp = tempfile.mkdtemp(prefix="sep_phys_")
+    syms, xyz = decks.cdse_dot(4.0)
+    decks.write_deck(tmp, syms, xyz, n, n, n, ms, ns, nc, lo, hi, pots={})
+    for s in ("Cd", "Se"):
+        shutil.copy(os.path.join(ROOT, "tests", "golden", "In13_P16", f"pot{s}.par"), tmp)
+    d = read_deck(tmp)
+    ctx = sep.Context(n, n, n, d["dx"], d["dy"], d["dz"], d["xmin"], d["ymin"], d["zmin"])
+    ctx.build_potential(d["xyz"], d["type"], d["npot"], d["vr"], d["pot"], d["dr"],
+                        expot=d["expot"])
+    dE, _, _ = ctx.energy_range()
+    zn, _ = sep.leja_points(nc)
+    an = sep.newton_coeffs(zn, np.linspace(lo, hi, ms), dE, ctx.vmin, (nc / (2.5 * dE)) ** 2)
+    N = n ** 3
+    psi0 = np.empty((ns, N))
+    idum, rng = -1001, None
+    for j in range(ns):
+        psi0[j], idum, rng = sep.init_psi(idum, N, d["dv"], rng)
+
+    def edge(ev, sg):
+        ok = np.sort(ev[(sg < 1e-2) & (ev >= lo) & (ev <= hi)])
+        i = int(np.argmax(np.diff(ok)))
+        return float(ok[i]), float(ok[i + 1])
+
+    t0 = time.perf_counter()
+    ctx.filter(psi0, an, zn, ctx.vmin, dE, sep.SEP_PREC_FP64, want_states=False)
+    k64, _ = ctx.ortho(ns * ms)
+    e64 = ctx.rayleigh_ritz(k64)
+    s64 = ctx.sigma(k64)
+    t64 = time.perf_counter() - t0
+    ctx.trim()
+    t0 = time.perf_counter()
+    ctx.filter(psi0, an, zn, ctx.vmin, dE, sep.SEP_PREC_FP32, want_states=False)
+    k32, _ = ctx.ortho_fp32(ns * ms)
+    ev = ctx.rayleigh_ritz(k32)
+    sg = ctx.sigma(k32)
+    _, cbm = edge(ev, sg)
+    ev32, sg32 = ctx.refine_window(ev, sg, cbm - 0.1 * ev_ha, cbm + 1.1 * ev_ha, passes=10,
+                                   full=True)
+    t32 = time.perf_counter() - t0
+    ctx.close()
+    vbm64, cbm64 = edge(e64, s64)
+    out = {"config": "CdSe 4.0 nm dot on the reference's potCd/potSe tables, 128^3, window "
+                     "[-0.26, -0.12] Ha, 16 targets x 16 states, nc 4096, seed -1001",
+           "gap_eV": round((cbm64 - vbm64) / ev_ha, 4), "cbm_Ha": {"fp64": round(cbm64, 8),
+                                                                   "fp32": round(cbm, 8)},
+           "pipeline_s": {"fp64": round(t64, 3), "fp32": round(t32, 3)}}
+    try:
+        nclu, nl, dmax = levels.compare_levels(e64, s64, ev32, sg32, cbm64 - levels.MEV,
+                                               cbm64 + ev_ha, levels.MEV)
+        out.update(conduction_levels=nl, multiplets=nclu, max_abs_err_meV=round(dmax / levels.MEV, 6),
+                   passed=nl > 0, matching="one to one per multiplet")
+    except AssertionError as exc:
+        out.update(passed=False, failure=str(exc)[:300])
+    return out
+
+
 def edge_gate(np, decks, levels, e64, s64, e32, s32, conv=1e-4):
     """The 1 meV gate, one to one: every FP64 level with sigma < conv within
     0.5 eV of the lowest such level, multiplet by multiplet, against the
@@ -491,10 +561,12 @@ def main():
         adj[label + "_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     log(f"hf adjoint: {adj}")
 
-    check = None
+    check = phys = None
     if not args.no_check:
         check = precision_check(sep, decks, read_deck, np)
         log(f"fp32 pipeline check: {check}")
+        phys = physical_check(sep, decks, read_deck, np)
+        log(f"fp32 pipeline at a real band edge: {phys}")
 
     # ---------------------------------------------------------- CPU baseline
     cpu = None
@@ -571,6 +643,7 @@ def main():
         "accum_ms_per_block": {"fp32": round(live32[3], 4),
                                "fp64": round(live64[3], 4) if live64 is not None else None},
         "fp32_pipeline_check": check,
+        "fp32_physical_edge_check": phys,
         "pipeline_128": pipe or None,
         # the whole pipeline at this workload (filter, ortho, Rayleigh-Ritz,
         # sigma; FP32: plus the FP64 correction passes of the band-edge window)
