@@ -562,6 +562,9 @@ def main():
     log(f"hf adjoint: {adj}")
 
     check = phys = None
+    # the timed filters are done: the main workload's ring of Newton vectors
+    # (up to 60 % of the free memory) would otherwise shrink the checks' rings
+    ctx.trim()
     if not args.no_check:
         check = precision_check(sep, decks, read_deck, np)
         log(f"fp32 pipeline check: {check}")
