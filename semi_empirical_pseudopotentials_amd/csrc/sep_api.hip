@@ -757,7 +757,7 @@ static int zero_dev(void *p, size_t bytes, hipStream_t st) {
 template <typename R>
 static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool psi0_on_device,
                         long ns, const double *an, const double *zn, long nc, long ms, double vmin,
-                        double dE, bool so = false) {
+                        double dE, bool so = false, double ring_cap = 0.0) {
   const long N = c->gd.N;
   if (!c->has_pot) return fail(c, SEP_ERR_ARG, "potential not set");
   f.c = c;
@@ -776,9 +776,11 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
   // at 128^3 with 16 states, 1024 complex64 vectors (137 GB) filter 4 %
   // faster than 512.  The test hook SEP_RING_DEPTH lowers the bound on K, so
   // that short filters span several ring blocks.
+  // ring_cap > 0: the bound a batch of grids planned for this one
   size_t mfree = 0, mtotal = 0;
-  if (hipMemGetInfo(&mfree, &mtotal) != hipSuccess) mfree = 0;
-  const double cap = std::min(160.0e9, 0.6 * ((double)mfree + (double)c->ring_bytes));
+  if (ring_cap <= 0.0 && hipMemGetInfo(&mfree, &mtotal) != hipSuccess) mfree = 0;
+  const double cap =
+      ring_cap > 0.0 ? ring_cap : std::min(160.0e9, 0.6 * ((double)mfree + (double)c->ring_bytes));
   const char *kenv = getenv("SEP_RING_DEPTH");
   const long kmax = kenv && atol(kenv) >= 1 && atol(kenv) <= 1024 ? atol(kenv) : 1024;
   long K = nc < kmax ? nc : kmax;
@@ -1015,9 +1017,41 @@ static int filter_batch_impl(int nd, sep_ctx *const *ctxs, const void *const *ps
   std::vector<hipGraphExec_t> execs(nd, nullptr);
   std::vector<hipGraph_t> graphs(nd, nullptr);
   int rc = SEP_OK;
+  // One ring depth K for all grids: the rings share 60 % of the free memory
+  // (the rings these contexts hold count as free) in proportion to their
+  // bytes per Newton vector, instead of each taking 60 % of what the grids
+  // before it left.  Rings larger than planned are released first, so the
+  // planned ones fit.
+  std::vector<double> per_k(nd);
+  double avail = 0.0, sum_k = 0.0;
+  {
+    size_t mfree = 0, mtotal = 0;
+    if (hipMemGetInfo(&mfree, &mtotal) == hipSuccess) avail = (double)mfree;
+  }
+  long kmax = 0;
+  for (int d = 0; d < nd; d++) {
+    per_k[d] = (double)(2 * ((ns[d] + 1) / 2)) * (double)ctxs[d]->gd.N * sizeof(R);
+    sum_k += per_k[d];
+    avail += (double)ctxs[d]->ring_bytes;
+    kmax = std::max(kmax, nc[d]);
+  }
+  const char *kenv = getenv("SEP_RING_DEPTH");
+  long K = std::min(kmax, (kenv && atol(kenv) >= 1 && atol(kenv) <= 1024) ? atol(kenv) : 1024L);
+  const double total = std::min(160.0e9 * nd, 0.6 * avail);
+  while (K > 8 && (double)K * sum_k > total) K /= 2;
+  for (int d = 0; d < nd; d++) {
+    sep_ctx *c = ctxs[d];
+    if (c->ring && (double)c->ring_bytes > (double)K * per_k[d]) {
+      hipStreamSynchronize(c->st);
+      hipFree(c->ring);
+      c->ring = nullptr;
+      c->ring_bytes = 0;
+    }
+  }
   for (int d = 0; d < nd && rc == SEP_OK; d++) {
     rc = filter_start<R>(runs[d], ctxs[d], static_cast<const double *>(psi0[d]), true, ns[d],
-                         an[d], zn[d], nc[d], ms[d], vmin[d], dE[d]);
+                         an[d], zn[d], nc[d], ms[d], vmin[d], dE[d], false,
+                         (double)K * per_k[d] * (1.0 + 1e-9));
     if (rc == SEP_OK) rc = hpsi_prepare<R>(ctxs[d]->gd);
   }
   if (rc == SEP_OK && mode == SEP_BATCH_GRAPH) {
