@@ -205,7 +205,6 @@ def test_physical_cdse_conduction_edge_fp32_vs_fp64(sep, tmp_path):
         k64, _ = ctx.ortho(ns * ms)
         e64 = ctx.rayleigh_ritz(k64)
         s64 = ctx.sigma(k64)
-        ctx.trim()
         print("case physical CdSe complex64 pipeline", flush=True)
         ctx.filter(psi0, an, zn, ctx.vmin, dE, sep.SEP_PREC_FP32, want_states=False)
         k32, _ = ctx.ortho_fp32(ns * ms)
