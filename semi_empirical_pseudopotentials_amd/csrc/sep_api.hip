@@ -334,7 +334,8 @@ int sep_close(sep_ctx *c) {
   for (hipEvent_t e : c->tev)
     if (e) hipEventDestroy(e);
   for (void *p : {(void *)c->so.off, (void *)c->so.jxyz, (void *)c->so.geo, (void *)c->so.pidx,
-                  (void *)c->so.poff, (void *)c->so.pent, (void *)c->so_contrib})
+                  (void *)c->so.poff, (void *)c->so.pent, (void *)c->so.kmat, (void *)c->so.koff,
+                  (void *)c->so_contrib})
     if (p) hipFree(p);
   if (c->blas) rocblas_destroy_handle(c->blas);
   if (c->dws) hipFree(c->dws);
@@ -2507,7 +2508,7 @@ int sep_so_setup(sep_ctx *c, long natom, const double *xyz, const double *vso, l
   const double d[3] = {c->dx, c->dy, c->dz}, x0[3] = {c->xmin, c->ymin, c->zmin};
   RCHK(so_build_host(c->gd.nx, c->gd.ny, c->gd.nz, d, x0, natom, xyz, vso, h));
   for (void *p : {(void *)c->so.off, (void *)c->so.jxyz, (void *)c->so.geo, (void *)c->so.pidx,
-                  (void *)c->so.poff, (void *)c->so.pent})
+                  (void *)c->so.poff, (void *)c->so.pent, (void *)c->so.kmat, (void *)c->so.koff})
     if (p) hipFree(p);
   c->so = SoLists();
   SoLists &s = c->so;
@@ -2532,6 +2533,8 @@ int sep_so_setup(sep_ctx *c, long natom, const double *xyz, const double *vso, l
   RCHK(up((void **)&s.pidx, h.pidx.data(), sizeof(int) * h.pidx.size()));
   RCHK(up((void **)&s.poff, h.poff.data(), sizeof(long) * h.poff.size()));
   RCHK(up((void **)&s.pent, h.pent.data(), sizeof(int) * h.pent.size()));
+  if (so_kmat_build(s, h.off, c->st) != SEP_OK) return fail(c, SEP_ERR_HIP, "so_setup: K matrices");
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return SEP_OK;
 }
 

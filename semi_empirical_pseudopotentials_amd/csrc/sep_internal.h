@@ -48,6 +48,10 @@ struct SoLists {
   double c = 0.0;         // dv / (2 sqrt(2 pi sigma^2)), hamiltonian.c:98
   double s1 = 0.0;        // sigma_1 = sqrt(1/2) / sigma, init.c:97
   long maxn = 0;          // longest list
+  // per atom the n x n matrix K(p, q) of the pair kernel (k_so_kmat), row q
+  // at kmat + koff[a] + q n; null where it would take more than 8 GB
+  double *kmat = nullptr;
+  long *koff = nullptr;
 };
 struct SoHost {
   std::vector<long> off, poff;
@@ -61,6 +65,9 @@ int so_build_host(int nx, int ny, int nz, const double d[3], const double x0[3],
 // in + s ss + c cs (out likewise); contrib holds nent * nsp * 4 doubles
 int so_apply_dev(const SoLists &so, const double *in, long cs, long ss, double *out, long ocs,
                  long oss, long nsp, double alpha, double *contrib, hipStream_t st);
+// Builds so.kmat / so.koff from the uploaded lists (off: the host copy of
+// so.off); leaves them null where they would exceed 8 GB.
+int so_kmat_build(SoLists &so, const std::vector<long> &off, hipStream_t st);
 int spinor_times_i_dev(const double *x, double *out, long N, long m, hipStream_t st);
 int spinor_gather_rows_dev(const double *src, double *dst, long N, long nsp, long ms,
                            hipStream_t st);
