@@ -49,6 +49,8 @@ def parse():
                         "parallel (about 20 s on 16 threads)")
     p.add_argument("--pipeline", action=argparse.BooleanOptionalAction, default=True,
                    help="time the whole filter-diagonalization pipeline of both precisions")
+    p.add_argument("--no-spinor", action="store_true",
+                   help="skip the spin-orbit spinor filter workload")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched uneven-grid measurement (8 dots, 64-160^3)")
     p.add_argument("--no-check", action="store_true",
@@ -59,6 +61,9 @@ def parse():
     p.add_argument("--kernels-only", action="store_true",
                    help="only launch the H.psi passes (FP32, 5 times) and a 256 MiB device "
                         "copy that calibrates the HBM byte counters; for rocprofv3 --pmc runs")
+    p.add_argument("--spinor-only", action="store_true",
+                   help="only run the spin-orbit spinor filter workload (InAs 3.0 nm dot, "
+                        "96^3); for rocprofv3 runs")
     return p.parse_args()
 
 
@@ -233,6 +238,80 @@ def physical_check(sep, decks, read_deck, np):
     return out
 
 
+def spinor_bench(sep, decks, np, torch, nc=1024, ns=8, ms=16, cpu_s=10.0, cpu=True):
+    """The spin-orbit filter (spinors/filterSpinors, FP64) on a 3.0 nm InAs
+    dot at 96^3 (0.67 bohr): In and As on zincblende sites (the reference's Vso values,
+    read.c:36,51), P1/P2 passivation (Vso 0), synthetic radial tables,
+    Ekinmax 20 Ha; ns random spinors x ms targets, Newton length nc.
+    Throughput in spinor H.psi applications per second (ns (nc - 1) / t,
+    inputs resident), and the oracle's spinor filter (numpy FFT, one thread)
+    timed on a bounded sample of the same Newton steps as the CPU figure."""
+    n = 96
+    syms, xyz = decks.cdse_dot(3.0, lattice_A=6.057)
+    syms = [{"Cd": "In", "Se": "As"}.get(s, s) for s in syms]
+    a = 6.057 / 0.529177
+    vso = np.array([0.48 * a ** 3 / 8.0 / 27.2114 if s == "In" else
+                    (3.15164354 if s == "As" else 0.0) for s in syms])
+    xyz = xyz - xyz.mean(axis=0)
+    half, d = decks.box_from_atoms(xyz, n, n, n)
+    tabs = {"In": (-3.0, 1.5, 0.0, 1.0), "As": (-2.5, 1.4, 0.0, 1.0), "P": (-0.5, 1.0, 0.0, 1.0)}
+    stride = 8192
+    vr = np.zeros((3, stride))
+    pot = np.zeros((3, stride))
+    npot, dr = [], []
+    for t, key in enumerate(("In", "As", "P")):
+        r, v = decks.pot_table(*tabs[key])
+        vr[t, :r.size], pot[t, :r.size] = r, v - v[-1]
+        npot.append(r.size)
+        dr.append(r[1] - r[0])
+    types = np.array([0 if s == "In" else (1 if s == "As" else 2) for s in syms])
+    ctx = sep.Context(n, n, n, d[0], d[1], d[2], -half[0], -half[1], -half[2], ekinmax=20.0)
+    V = ctx.build_potential(xyz, types, npot, vr, pot, dr)
+    nent = ctx.so_setup(xyz, vso)
+    vmin, dE, its = ctx.energy_range_spinor()
+    zn, _ = sep.leja_points(nc)
+    e0 = vmin / 1.2 if vmin < 0 else vmin / 0.8
+    an = sep.newton_coeffs(zn, np.linspace(e0, e0 + 0.4, ms), dE, vmin, (nc / (2.5 * dE)) ** 2)
+    N = n ** 3
+    g = torch.Generator(device="cuda:0").manual_seed(3)
+    psi0 = torch.zeros((ns, 4, N), dtype=torch.float64, device="cuda:0")
+    psi0[:, 0].uniform_(-1.0, 1.0, generator=g)
+    psi0[:, 2].uniform_(-1.0, 1.0, generator=g)
+    psi0 *= (1.0 / torch.sqrt((psi0[:, 0] ** 2).sum(dim=1) * float(np.prod(d))))[:, None, None]
+    ctx.filter_spinor_dev(psi0.data_ptr(), ns, an, zn, vmin, dE)  # warm-up (ring allocation)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.filter_spinor_dev(psi0.data_ptr(), ns, an, zn, vmin, dE)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ctx.close()
+    out = {"config": f"InAs 3.0 nm dot, {len(syms)} sites ({int((vso > 0).sum())} with Vso), "
+                     f"{n}^3, Ekinmax 20, {ns} spinors x {ms} targets, nc {nc}, FP64",
+           "so_entries": int(nent), "dE": round(float(dE), 4),
+           "filter_s": round(t, 4), "spinor_hpsi_per_s": round(ns * (nc - 1) / t, 1)}
+    if not cpu:
+        return out
+    # CPU: the oracle's spinor H.psi on one spinor, Newton steps until cpu_s
+    from oracle import oracle as O
+    from oracle import spinor as S
+    grid = (n, n, n)
+    ksq = O.ksqr(n, n, n, d[0], d[1], d[2], 20.0)
+    lists = S.so_lists(grid, tuple(-half), tuple(d), xyz, vso)
+    psi = psi0[0].cpu().numpy()
+    z = np.stack([psi[0] + 1j * psi[1], psi[2] + 1j * psi[3]])
+    dv = float(np.prod(d))
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < cpu_s:
+        z = S.hamiltonian(grid, ksq, V, lists, z, dv, d[0]) * (4.0 / dE) - z
+        steps += 1
+    tc = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": round(steps / tc, 3), "unit": "spinor H.psi/s", "cores": 1,
+                           "kind": "port",
+                           "sample": f"oracle spinor H (numpy FFT + matrix-form V_so), one spinor, "
+                                     f"{steps} Newton-step H applications in {tc:.1f} s"}
+    return out
+
+
 def edge_gate(np, decks, levels, e64, s64, e32, s32, conv=1e-4):
     """The 1 meV gate, one to one: every FP64 level with sigma < conv within
     0.5 eV of the lowest such level, multiplet by multiplet, against the
@@ -345,6 +424,10 @@ def main():
     n, ns, ms, nc = args.grid, args.ns, args.ms, args.nc
     if args.gemm_only:
         print(json.dumps({"gemm_only": True, **subspace_gemms(sep, np, torch, n)}), flush=True)
+        return
+    if args.spinor_only:
+        print(json.dumps({"spinor_only": True,
+                          **spinor_bench(sep, decks, np, torch, cpu=not args.no_cpu)}), flush=True)
         return
     # ---------------------------------------------------------------- setup
     t_setup = time.time()
@@ -607,6 +690,10 @@ def main():
     if ctx is not None:
         ctx.close()
         ctx = None
+    spinor = None
+    if not args.no_spinor:
+        spinor = spinor_bench(sep, decks, np, torch, cpu=not args.no_cpu)
+        log(f"spinor filter: {spinor}")
     gemm = subspace_gemms(sep, np, torch, n)
     log(f"subspace gemms: {gemm}")
 
@@ -651,6 +738,7 @@ def main():
         "gate_1meV_one_to_one": (pipe.get("edge_levels_fp32_vs_fp64", {}).get("passed")
                                  if pipe else None),
         "batched_uneven": batched,
+        "spinor_filter": spinor,
         "hf_adjoint": dict(adj, atoms=d["natom"], states=m),
         "subspace_gemm": gemm,
         "cpu_baseline": cpu,
