@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "sep_internal.h"
@@ -159,7 +160,9 @@ __global__ void __launch_bounds__(SO_T) k_so_contrib_k(SoLists so, const double 
   if (a >= so.natom) return;  // padding blocks of the last group of eight
   const long o0 = so.off[a], n = so.off[a + 1] - o0;
   const double *km = so.kmat + so.koff[a];
-  __shared__ double qw[NSB][12][SO_QT];
+  // the 12 sums of one q side by side: the inner loop reads them as one
+  // broadcast row (wide LDS reads) per K
+  __shared__ __align__(16) double qw[NSB][SO_QT][12];
   const double c = so.c;
   for (long p0 = 0; p0 < n; p0 += SO_T) {
     const long p = p0 + threadIdx.x;
@@ -185,10 +188,10 @@ __global__ void __launch_bounds__(SO_T) k_so_contrib_k(SoLists so, const double 
 #pragma unroll
             for (int m = 0; m < 3; m++) {
               const double yr = g[yo[m]], yi = g[yo[m] + 1];
-              qw[u][2 * m][t] = yr * ur + yi * ui;
-              qw[u][2 * m + 1][t] = yr * ui - yi * ur;
-              qw[u][6 + 2 * m][t] = yr * dr + yi * di;
-              qw[u][6 + 2 * m + 1][t] = yr * di - yi * dr;
+              qw[u][t][2 * m] = yr * ur + yi * ui;
+              qw[u][t][2 * m + 1] = yr * ui - yi * ur;
+              qw[u][t][6 + 2 * m] = yr * dr + yi * di;
+              qw[u][t][6 + 2 * m + 1] = yr * di - yi * dr;
             }
           }
         }
@@ -203,7 +206,7 @@ __global__ void __launch_bounds__(SO_T) k_so_contrib_k(SoLists so, const double 
 #pragma unroll
           for (int u = 0; u < NSB; u++)
 #pragma unroll
-            for (int k = 0; k < 12; k++) S[u][k] += K * qw[u][k][t];
+            for (int k = 0; k < 12; k++) S[u][k] += K * qw[u][t][k];
         }
       }
     }
@@ -241,6 +244,8 @@ int so_kmat_build(SoLists &so, const std::vector<long> &off, hipStream_t st) {
   }
   const size_t bytes = sizeof(double) * (size_t)koff[so.natom];
   if (bytes > ((size_t)8 << 30)) return SEP_OK;  // the inline kernel instead
+  const char *env = getenv("SEP_SO_KMAT");  // 0: always the inline kernel
+  if (env && atoi(env) == 0) return SEP_OK;
   if (hipMalloc(&so.kmat, bytes) != hipSuccess) {
     (void)hipGetLastError();
     so.kmat = nullptr;

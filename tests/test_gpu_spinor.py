@@ -74,6 +74,28 @@ def test_spinor_h_vs_oracle(sep, oracle):
     assert err < 1e-12
 
 
+def test_spinor_h_kmat_matches_inline(sep, oracle, monkeypatch):
+    """The set-up's per-atom matrices of the pair kernel K(p, q) and the
+    inline kernel (used when the matrices exceed 8 GB or do not fit) give the
+    same V_so psi bit for bit: one expression, one q order."""
+    import torch
+    n, d = 24, 0.5
+    rng = np.random.default_rng(12)
+    psi = rng.standard_normal((5, 4 * n ** 3))
+    out = []
+    for knob in ("1", "0"):
+        print("case spinor H 24^3, SEP_SO_KMAT", knob, flush=True)
+        monkeypatch.setenv("SEP_SO_KMAT", knob)
+        _, _, _, _, _, ctx = _setup(sep, oracle, n, d)
+        with ctx:
+            x = torch.from_numpy(psi).to("cuda:0")
+            y = torch.empty_like(x)
+            ctx.apply_h_spinor_dev(x.data_ptr(), y.data_ptr(), 5)
+            out.append(y.cpu().numpy())
+    assert np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0], out[1])
+
+
 def test_spinor_pipeline_vs_oracle(sep, oracle):
     import torch
     n, d = 16, 0.6
