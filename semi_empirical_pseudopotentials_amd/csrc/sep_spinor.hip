@@ -31,6 +31,55 @@ namespace sep {
 static constexpr int SO_T = 256;   // threads per workgroup
 static constexpr int SO_QT = 128;  // q points per LDS tile
 
+// The rounding of the spin-orbit term is spelled out (explicit fma, no
+// other contraction), so that the kernel that evaluates the pair kernel
+// inline and the one that reads it precomputed agree bit for bit.
+
+// K(p, q) = (V_p + V_q)/2 exp(-((r_p - r_q) s1)^2) / ((r2_p + r2_q)/2)
+__device__ __forceinline__ double so_pair(double rp, double r2p, double vp, double rq, double r2q,
+                                          double vq, double s1) {
+#pragma clang fp contract(off)
+  const double rr = (rp - rq) * s1;
+  return 0.5 * (vp + vq) * exp(-(rr * rr)) / (0.5 * (r2p + r2q));
+}
+
+// The 12 products conj(Y_m) psi_s of one point (g: its 9 geometry words;
+// y11 at g[5], y10 at g[3], y1-1 at g[7]): w[6 s + 2 m + {re, im}], m = 11,
+// 10, 1-1; s = up, dn.
+__device__ __forceinline__ void so_point_w(const double *g, double ur, double ui, double dr,
+                                           double di, double *w) {
+#pragma clang fp contract(off)
+  const int yo[3] = {5, 3, 7};
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    const double yr = g[yo[m]], yi = g[yo[m] + 1];
+    w[2 * m] = fma(yr, ur, yi * ui);
+    w[2 * m + 1] = fma(yr, ui, -(yi * ur));
+    w[6 + 2 * m] = fma(yr, dr, yi * di);
+    w[6 + 2 * m + 1] = fma(yr, di, -(yi * dr));
+  }
+}
+
+// The four outputs (up re, up im, dn re, dn im) of point p from its 12 sums
+// S (hamiltonian.c:160-185).
+__device__ __forceinline__ void so_point_out(const double *g, double c, const double *S,
+                                             double *o) {
+#pragma clang fp contract(off)
+  const double t10r = c * g[3], t10i = c * g[4];
+  const double t11r = c * g[5], t11i = c * g[6];
+  const double t1_r = c * g[7], t1_i = c * g[8];
+  const double s2 = 1.4142135623730950488;
+  // S index: up 11 = 0, up 10 = 2, up 1-1 = 4; dn 11 = 6, dn 10 = 8, dn 1-1 = 10
+#define CRE(tr, ti, k) fma((tr), S[k], -((ti) * S[(k) + 1]))
+#define CIM(tr, ti, k) fma((ti), S[k], (tr) * S[(k) + 1])
+  o[0] = fma(s2, CRE(t10r, t10i, 6) + CRE(t1_r, t1_i, 8), CRE(t11r, t11i, 0) - CRE(t1_r, t1_i, 4));
+  o[1] = fma(s2, CIM(t10r, t10i, 6) + CIM(t1_r, t1_i, 8), CIM(t11r, t11i, 0) - CIM(t1_r, t1_i, 4));
+  o[2] = fma(s2, CRE(t10r, t10i, 4) + CRE(t11r, t11i, 2), -CRE(t11r, t11i, 6) - CRE(t1_r, t1_i, 10));
+  o[3] = fma(s2, CIM(t10r, t10i, 4) + CIM(t11r, t11i, 2), -CIM(t11r, t11i, 6) - CIM(t1_r, t1_i, 10));
+#undef CRE
+#undef CIM
+}
+
 __global__ void __launch_bounds__(SO_T) k_so_contrib(SoLists so, const double *in, long cs, long ss,
                                                      double *contrib, long nsp) {
   const int a = blockIdx.x;
@@ -38,7 +87,7 @@ __global__ void __launch_bounds__(SO_T) k_so_contrib(SoLists so, const double *i
   const long o0 = so.off[a], n = so.off[a + 1] - o0;
   const double *ps = in + s * ss;
   __shared__ double qr[SO_QT], qr2[SO_QT], qv[SO_QT];
-  __shared__ double qw[12][SO_QT];
+  __shared__ __align__(16) double qw[SO_QT][12];
   const double s1 = so.s1, c = so.c;
   for (long p0 = 0; p0 < n; p0 += SO_T) {
     const long p = p0 + threadIdx.x;
@@ -64,47 +113,20 @@ __global__ void __launch_bounds__(SO_T) k_so_contrib(SoLists so, const double *i
           qr[t] = g[0];
           qr2[t] = g[1];
           qv[t] = g[2];
-          const double ur = ps[j], ui = ps[cs + j], dr = ps[2 * cs + j], di = ps[3 * cs + j];
-          // conj(y) psi for y11 (g[5], g[6]), y10 (g[3], g[4]), y1-1 (g[7], g[8])
-          const int yo[3] = {5, 3, 7};
-#pragma unroll
-          for (int m = 0; m < 3; m++) {
-            const double yr = g[yo[m]], yi = g[yo[m] + 1];
-            qw[2 * m][t] = yr * ur + yi * ui;
-            qw[2 * m + 1][t] = yr * ui - yi * ur;
-            qw[6 + 2 * m][t] = yr * dr + yi * di;
-            qw[6 + 2 * m + 1][t] = yr * di - yi * dr;
-          }
+          so_point_w(g, ps[j], ps[cs + j], ps[2 * cs + j], ps[3 * cs + j], qw[t]);
         }
       }
       __syncthreads();
       const int qn = (int)(n - q0 < SO_QT ? n - q0 : SO_QT);
       if (act) {
         for (int t = 0; t < qn; t++) {
-          const double rr = (rp - qr[t]) * s1;
-          const double K = 0.5 * (vp + qv[t]) * exp(-(rr * rr)) / (0.5 * (r2p + qr2[t]));
+          const double K = so_pair(rp, r2p, vp, qr[t], qr2[t], qv[t], s1);
 #pragma unroll
-          for (int k = 0; k < 12; k++) S[k] += K * qw[k][t];
+          for (int k = 0; k < 12; k++) S[k] = fma(K, qw[t][k], S[k]);
         }
       }
     }
-    if (act) {
-      const double *g = so.geo + (o0 + p) * 9;
-      const double t10r = c * g[3], t10i = c * g[4];
-      const double t11r = c * g[5], t11i = c * g[6];
-      const double t1_r = c * g[7], t1_i = c * g[8];
-      const double s2 = 1.4142135623730950488;
-      // S index: up 11 = 0, up 10 = 2, up 1-1 = 4; dn 11 = 6, dn 10 = 8, dn 1-1 = 10
-#define CRE(tr, ti, k) ((tr) * S[k] - (ti) * S[(k) + 1])
-#define CIM(tr, ti, k) ((ti) * S[k] + (tr) * S[(k) + 1])
-      double *o = contrib + ((o0 + p) * nsp + s) * 4;
-      o[0] = CRE(t11r, t11i, 0) - CRE(t1_r, t1_i, 4) + s2 * (CRE(t10r, t10i, 6) + CRE(t1_r, t1_i, 8));
-      o[1] = CIM(t11r, t11i, 0) - CIM(t1_r, t1_i, 4) + s2 * (CIM(t10r, t10i, 6) + CIM(t1_r, t1_i, 8));
-      o[2] = -CRE(t11r, t11i, 6) - CRE(t1_r, t1_i, 10) + s2 * (CRE(t10r, t10i, 4) + CRE(t11r, t11i, 2));
-      o[3] = -CIM(t11r, t11i, 6) - CIM(t1_r, t1_i, 10) + s2 * (CIM(t10r, t10i, 4) + CIM(t11r, t11i, 2));
-#undef CRE
-#undef CIM
-    }
+    if (act) so_point_out(so.geo + (o0 + p) * 9, c, S, contrib + ((o0 + p) * nsp + s) * 4);
   }
 }
 
@@ -138,99 +160,78 @@ __global__ void __launch_bounds__(SO_T) k_so_kmat(SoLists so) {
     const double rp = gp[0], r2p = gp[1], vp = gp[2];
     for (long q = 0; q < n; q++) {
       const double *gq = so.geo + (o0 + q) * 9;
-      const double rr = (rp - gq[0]) * s1;
-      km[q * n + p] = 0.5 * (vp + gq[2]) * exp(-(rr * rr)) / (0.5 * (r2p + gq[1]));
+      km[q * n + p] = so_pair(rp, r2p, vp, gq[0], gq[1], gq[2], s1);
     }
   }
 }
 
-// k_so_contrib with K read from the matrices of k_so_kmat, NSB spinors per
-// workgroup (each K read serves 12 NSB accumulations).  A 1-D grid over
-// ceil(natom / 8) * 8 * ceil(nsp / NSB) blocks: block b runs on XCD b % 8
-// (round-robin dispatch), and the blocks of one atom share that XCD and sit
-// close in the dispatch order, so the atom's K is read from HBM once and
-// from that XCD's L2 by its other blocks.
-template <int NSB>
-__global__ void __launch_bounds__(SO_T) k_so_contrib_k(SoLists so, const double *in, long cs,
-                                                       long ss, double *contrib, long nsp) {
-  const long nsg = (nsp + NSB - 1) / NSB;  // spinor groups
+// k_so_contrib with K read from the matrices of k_so_kmat: KT threads, NP
+// points p per thread.  A 1-D grid over ceil(natom / 8) * 8 * nsp blocks:
+// block b runs on XCD b % 8 (round-robin dispatch), and the blocks of one
+// atom share that XCD and sit close in the dispatch order, so the atom's K
+// is read from HBM once and from that XCD's L2 by its other spinors.  The
+// kernel is bound by that stream of K: at 96^3 with 465 atoms (244-277
+// points each) KT = 256, NP = 1 takes 236 us per call on 8 spinors, and
+// KT = 128, NP = 2 (half the LDS bytes per FMA, but more atoms in flight
+// per XCD) 348 us.  Each point's sums run over q in the order of
+// k_so_contrib: the two kernels agree bit for bit.
+template <int KT, int NP>
+__global__ void __launch_bounds__(KT) k_so_contrib_k(SoLists so, const double *in, long cs,
+                                                     long ss, double *contrib, long nsp) {
   const long b = blockIdx.x;
   const long x = b % 8, j = b / 8;
-  const long sg = j % nsg, a = (j / nsg) * 8 + x;
+  const long s = j % nsp, a = (j / nsp) * 8 + x;
   if (a >= so.natom) return;  // padding blocks of the last group of eight
   const long o0 = so.off[a], n = so.off[a + 1] - o0;
   const double *km = so.kmat + so.koff[a];
-  // the 12 sums of one q side by side: the inner loop reads them as one
-  // broadcast row (wide LDS reads) per K
-  __shared__ __align__(16) double qw[NSB][SO_QT][12];
+  const double *ps = in + s * ss;
+  __shared__ __align__(16) double qw[SO_QT][12];
   const double c = so.c;
-  for (long p0 = 0; p0 < n; p0 += SO_T) {
-    const long p = p0 + threadIdx.x;
-    const bool act = p < n;
-    double S[NSB][12];
+  for (long p0 = 0; p0 < n; p0 += KT * NP) {
+    // points past the list read row entries of the last point (in bounds)
+    // and are not written
+    long pc[NP];
 #pragma unroll
-    for (int u = 0; u < NSB; u++)
+    for (int i = 0; i < NP; i++) {
+      const long p = p0 + threadIdx.x + i * KT;
+      pc[i] = p < n ? p : n - 1;
+    }
+    double S[NP][12];
 #pragma unroll
-      for (int k = 0; k < 12; k++) S[u][k] = 0.0;
+    for (int i = 0; i < NP; i++)
+#pragma unroll
+      for (int k = 0; k < 12; k++) S[i][k] = 0.0;
     for (long q0 = 0; q0 < n; q0 += SO_QT) {
       __syncthreads();  // the previous tile is consumed
-      for (int t = threadIdx.x; t < SO_QT; t += SO_T) {
+      for (int t = threadIdx.x; t < SO_QT; t += KT) {
         const long q = q0 + t;
         if (q < n) {
           const double *g = so.geo + (o0 + q) * 9;
           const long jq = so.jxyz[o0 + q];
-          const int yo[3] = {5, 3, 7};
-#pragma unroll
-          for (int u = 0; u < NSB; u++) {
-            const long s = sg * NSB + u;
-            const double *ps = in + (s < nsp ? s : nsp - 1) * ss;
-            const double ur = ps[jq], ui = ps[cs + jq], dr = ps[2 * cs + jq], di = ps[3 * cs + jq];
-#pragma unroll
-            for (int m = 0; m < 3; m++) {
-              const double yr = g[yo[m]], yi = g[yo[m] + 1];
-              qw[u][t][2 * m] = yr * ur + yi * ui;
-              qw[u][t][2 * m + 1] = yr * ui - yi * ur;
-              qw[u][t][6 + 2 * m] = yr * dr + yi * di;
-              qw[u][t][6 + 2 * m + 1] = yr * di - yi * dr;
-            }
-          }
+          so_point_w(g, ps[jq], ps[cs + jq], ps[2 * cs + jq], ps[3 * cs + jq], qw[t]);
         }
       }
       __syncthreads();
       const int qn = (int)(n - q0 < SO_QT ? n - q0 : SO_QT);
-      if (act) {
-        const double *kq = km + q0 * n + p;
-#pragma unroll 8
-        for (int t = 0; t < qn; t++) {
-          const double K = kq[t * n];
+      const double *kq = km + q0 * n;
+#pragma unroll 4
+      for (int t = 0; t < qn; t++) {
+        double K[NP];
 #pragma unroll
-          for (int u = 0; u < NSB; u++)
+        for (int i = 0; i < NP; i++) K[i] = kq[t * n + pc[i]];
+        double w[12];
 #pragma unroll
-            for (int k = 0; k < 12; k++) S[u][k] += K * qw[u][t][k];
-        }
+        for (int k = 0; k < 12; k++) w[k] = qw[t][k];
+#pragma unroll
+        for (int i = 0; i < NP; i++)
+#pragma unroll
+          for (int k = 0; k < 12; k++) S[i][k] = fma(K[i], w[k], S[i][k]);
       }
     }
-    if (act) {
-      const double *g = so.geo + (o0 + p) * 9;
-      const double t10r = c * g[3], t10i = c * g[4];
-      const double t11r = c * g[5], t11i = c * g[6];
-      const double t1_r = c * g[7], t1_i = c * g[8];
-      const double s2 = 1.4142135623730950488;
 #pragma unroll
-      for (int u = 0; u < NSB; u++) {
-        const long s = sg * NSB + u;
-        if (s >= nsp) break;
-        const double *Su = S[u];
-#define CRE(tr, ti, k) ((tr) * Su[k] - (ti) * Su[(k) + 1])
-#define CIM(tr, ti, k) ((ti) * Su[k] + (tr) * Su[(k) + 1])
-        double *o = contrib + ((o0 + p) * nsp + s) * 4;
-        o[0] = CRE(t11r, t11i, 0) - CRE(t1_r, t1_i, 4) + s2 * (CRE(t10r, t10i, 6) + CRE(t1_r, t1_i, 8));
-        o[1] = CIM(t11r, t11i, 0) - CIM(t1_r, t1_i, 4) + s2 * (CIM(t10r, t10i, 6) + CIM(t1_r, t1_i, 8));
-        o[2] = -CRE(t11r, t11i, 6) - CRE(t1_r, t1_i, 10) + s2 * (CRE(t10r, t10i, 4) + CRE(t11r, t11i, 2));
-        o[3] = -CIM(t11r, t11i, 6) - CIM(t1_r, t1_i, 10) + s2 * (CIM(t10r, t10i, 4) + CIM(t11r, t11i, 2));
-#undef CRE
-#undef CIM
-      }
+    for (int i = 0; i < NP; i++) {
+      const long p = p0 + threadIdx.x + i * KT;
+      if (p < n) so_point_out(so.geo + (o0 + p) * 9, c, S[i], contrib + ((o0 + p) * nsp + s) * 4);
     }
   }
 }
@@ -263,12 +264,9 @@ int so_apply_dev(const SoLists &so, const double *in, long cs, long ss, double *
                  long oss, long nsp, double alpha, double *contrib, hipStream_t st) {
   if (so.natom == 0 || nsp < 1) return SEP_OK;
   if (so.kmat) {
-    // one spinor per workgroup: two (24 accumulators, 174 VGPRs, half the
-    // blocks) measured 476 against 243 us at 96^3 with 465 atoms
-    constexpr int NSB = 1;
-    const long nb = (long)((so.natom + 7) / 8) * 8 * ((nsp + NSB - 1) / NSB);
-    hipLaunchKernelGGL(k_so_contrib_k<NSB>, dim3((unsigned)nb), dim3(SO_T), 0, st, so, in, cs, ss,
-                       contrib, nsp);
+    const long nb = (long)((so.natom + 7) / 8) * 8 * nsp;
+    hipLaunchKernelGGL((k_so_contrib_k<256, 1>), dim3((unsigned)nb), dim3(256), 0, st, so, in, cs,
+                       ss, contrib, nsp);
   } else {
     hipLaunchKernelGGL(k_so_contrib, dim3((unsigned)so.natom, (unsigned)nsp), dim3(SO_T), 0, st,
                        so, in, cs, ss, contrib, nsp);
