@@ -67,6 +67,13 @@ static void die(const char *msg) {
   exit(1);
 }
 
+// host buffers: a failed allocation ends the program with a message
+static void *xmalloc(size_t bytes) {
+  void *p = malloc(bytes ? bytes : 1);
+  if (!p) die("out of host memory");
+  return p;
+}
+
 static double num_arg(const char *opt, const char *v) {
   char *end = NULL;
   const double x = strtod(v, &end);
@@ -95,8 +102,8 @@ static void couple_set(sep_ctx *ctx, const sep_deck *d, const qp_state *qs, long
   if (n < 1) return;
   /* pairs r <= s (the diagonal only for diabatic) */
   long np = 0;
-  long *pairs = (long *)malloc(sizeof(long) * 2 * n * (n + 1) / 2);
-  long *pidx = (long *)malloc(sizeof(long) * n * n); /* (r, s) -> pair, or -1 */
+  long *pairs = (long *)xmalloc(sizeof(long) * 2 * n * (n + 1) / 2);
+  long *pidx = (long *)xmalloc(sizeof(long) * n * n); /* (r, s) -> pair, or -1 */
   for (long r = 0; r < n; r++)
     for (long s = 0; s < n; s++) pidx[r * n + s] = -1;
   for (long r = 0; r < n; r++)
@@ -108,7 +115,7 @@ static void couple_set(sep_ctx *ctx, const sep_deck *d, const qp_state *qs, long
   const long na = d->natom;
   double *dR = NULL;
   if (np > 0) {
-    dR = (double *)malloc(sizeof(double) * np * na * 3);
+    dR = (double *)xmalloc(sizeof(double) * np * na * 3);
     if (!pairs || !pidx || !dR) die("couplings: out of memory");
     int rc = sep_hf_adjoint(ctx, np, pairs, NULL, na, d->xyz, d->type,
                             d->has_strain ? d->scale : NULL, SEP_NTYPE, d->npot, d->vr, d->pot,
@@ -191,13 +198,15 @@ int main(int argc, char **argv) {
   FILE *pf = dir_file(eval_name, "r");
   if (!pf) die("cannot open the eigenvalue file (eval.par)");
   long cap = 1024, nst = 0;
-  qp_state *all = (qp_state *)malloc(sizeof(qp_state) * cap);
+  qp_state *all = (qp_state *)xmalloc(sizeof(qp_state) * cap);
   long idx;
   double e, sg;
   while (fscanf(pf, "%ld %lg %lg", &idx, &e, &sg) == 3) {
     if (nst == cap) {
       cap *= 2;
-      all = (qp_state *)realloc(all, sizeof(qp_state) * cap);
+      qp_state *grown = (qp_state *)realloc(all, sizeof(qp_state) * cap);
+      if (!grown) die("out of host memory");
+      all = grown;
     }
     all[nst].index = nst;
     all[nst].e = e;
@@ -210,8 +219,8 @@ int main(int argc, char **argv) {
     if (all[i].e < all[i - 1].e) die("the eigenvalue file is not in ascending energy");
   /* holes: converged, below the Fermi energy, the max_holes highest;
    * electrons: converged, above it, the max_elecs lowest */
-  qp_state *holes = (qp_state *)malloc(sizeof(qp_state) * nst);
-  qp_state *elecs = (qp_state *)malloc(sizeof(qp_state) * nst);
+  qp_state *holes = (qp_state *)xmalloc(sizeof(qp_state) * nst);
+  qp_state *elecs = (qp_state *)xmalloc(sizeof(qp_state) * nst);
   long nh = 0, ne = 0;
   for (long i = nst - 1; i >= 0 && nh < max_holes; i--)
     if (all[i].sigma < sigma_cut && all[i].e < fermi) holes[nh++] = all[i];
@@ -232,7 +241,7 @@ int main(int argc, char **argv) {
   }
   if (nh + ne == 0) die("no converged states on either side of the Fermi energy");
   /* the selected states from the state file: holes then electrons */
-  double *psi = (double *)malloc(sizeof(double) * (nh + ne) * N);
+  double *psi = (double *)xmalloc(sizeof(double) * (nh + ne) * N);
   if (!psi) die("state buffer");
   pf = dir_file(psi_name, "rb");
   if (!pf) die("cannot open the state file (psi.par)");

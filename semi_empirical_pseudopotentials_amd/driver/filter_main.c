@@ -56,6 +56,13 @@ static void die(const char *msg) {
   exit(1);
 }
 
+// host buffers: a failed allocation ends the program with a message
+static void *xmalloc(size_t bytes) {
+  void *p = malloc(bytes ? bytes : 1);
+  if (!p) die("out of host memory");
+  return p;
+}
+
 static double wall(void) {
   struct timeval tv;
   gettimeofday(&tv, NULL);
@@ -286,10 +293,10 @@ int main(int argc, char **argv) {
 
   CHECK(sep_set_filter_lanes(ctx, lanes));
   /* V(r) on the device (filter/init.c:157-199) */
-  int *ti = (int *)malloc(sizeof(int) * d.natom);
+  int *ti = (int *)xmalloc(sizeof(int) * d.natom);
   for (long i = 0; i < d.natom; i++) ti[i] = d.type[i];
   double Vmin, Vmax;
-  double *Vhost = (double *)malloc(sizeof(double) * N);
+  double *Vhost = (double *)xmalloc(sizeof(double) * N);
   if (!Vhost) die("potential buffer");
   CHECK(sep_build_potential(ctx, d.natom, d.xyz, ti, d.has_strain ? d.scale : NULL, SEP_NTYPE,
                             d.npot, d.vr, d.pot, SEP_NPOT, d.dr, d.expot, Vhost, &Vmin, &Vmax));
@@ -305,7 +312,7 @@ int main(int argc, char **argv) {
   printf("dT = %g\n", 0.5 * PIE * PIE / (d.dx * d.dx) + 0.5 * PIE * PIE / (d.dy * d.dy) +
                           0.5 * PIE * PIE / (d.dz * d.dz));
   /* energy targets, filter/init.c:209-214 */
-  double *el = (double *)malloc(sizeof(double) * ms);
+  double *el = (double *)xmalloc(sizeof(double) * ms);
   {
     double del = ms > 1 ? (d.Elmax - d.Elmin) / (double)(ms - 1) : 0.0;
     FILE *pf = out_file("Egrid.dat", "w");
@@ -320,7 +327,7 @@ int main(int argc, char **argv) {
   double dE, Emax;
   long iters;
   {
-    double *elog = (double *)malloc(sizeof(double) * 3 * 500);
+    double *elog = (double *)xmalloc(sizeof(double) * 3 * 500);
     CHECK(sep_energy_range_log(ctx, Vmin, &dE, &Emax, &iters, elog, 500));
     FILE *pf = out_file("ene-init.dat", "w");  /* filter/energy.c:65-74 */
     if (pf) {
@@ -337,8 +344,8 @@ int main(int argc, char **argv) {
   const double dt = ((double)nc / (2.5 * dE)) * ((double)nc / (2.5 * dE));
   printf("nc = %ld dt = %g dE = %g\n", nc, dt, dE);
   fflush(stdout);
-  double *zn = (double *)malloc(sizeof(double) * nc);
-  double *an = (double *)malloc(sizeof(double) * nc * ms);
+  double *zn = (double *)xmalloc(sizeof(double) * nc);
+  double *an = (double *)xmalloc(sizeof(double) * nc * ms);
   sep_leja_points(nc, zn);
   if (sep_newton_coeffs(nc, ms, zn, el, dE, Vmin, dt, an) != SEP_OK) die("coefficient failed");
   FILE *pc = out_file("coeff.dat", "w");
@@ -347,7 +354,7 @@ int main(int argc, char **argv) {
   const long mstot = ms * ns;
   double *psi0 = NULL, *ene = NULL;
   if (resume) { /* the filtered states of an earlier run, ns files of ms states */
-    double *psitot = (double *)malloc(sizeof(double) * mstot * N);
+    double *psitot = (double *)xmalloc(sizeof(double) * mstot * N);
     if (!psitot) die("psi-filt buffer");
     for (long jns = 0; jns < ns; jns++) {
       char name[64];
@@ -377,13 +384,13 @@ int main(int argc, char **argv) {
     }
     printf("seed = %ld\n", idum);
     fflush(stdout);
-    psi0 = (double *)malloc(sizeof(double) * ns * N);
+    psi0 = (double *)xmalloc(sizeof(double) * ns * N);
     sep_rng rng;
     sep_rng_reset(&rng);
     for (long j = 0; j < ns; j++) sep_init_psi(&rng, &idum, N, d.dv, psi0 + j * N);
     /* filter (main.c:91-100) */
-    ene = (double *)malloc(sizeof(double) * mstot);
-    double *psif = psi_filt ? (double *)malloc(sizeof(double) * mstot * N) : NULL;
+    ene = (double *)xmalloc(sizeof(double) * mstot);
+    double *psif = psi_filt ? (double *)xmalloc(sizeof(double) * mstot * N) : NULL;
     t0 = wall();
     c0 = (double)clock();
     CHECK(sep_filter(ctx, psi0, ns, an, zn, nc, ms, Vmin, dE, prec, psif, ene));
@@ -422,7 +429,7 @@ int main(int argc, char **argv) {
   t0 = wall();
   c0 = (double)clock();
   long kept = 0;
-  double *S = (double *)malloc(sizeof(double) * mstot);
+  double *S = (double *)xmalloc(sizeof(double) * mstot);
   if (prec == SEP_PREC_FP32)
     CHECK(sep_ortho_fp32(ctx, mstot, 1.0e-10, S, &kept));
   else
@@ -434,8 +441,8 @@ int main(int argc, char **argv) {
   /* Rayleigh-Ritz (main.c:120-126) */
   t0 = wall();
   c0 = (double)clock();
-  double *eval = (double *)malloc(sizeof(double) * kept);
-  double *sige = (double *)malloc(sizeof(double) * kept);
+  double *eval = (double *)xmalloc(sizeof(double) * kept);
+  double *sige = (double *)xmalloc(sizeof(double) * kept);
   CHECK(sep_rayleigh_ritz(ctx, kept, eval));
   write_hmat_log(kept, N);
   if (prec == SEP_PREC_FP32) {
@@ -453,7 +460,7 @@ int main(int argc, char **argv) {
   fflush(stdout);
   /* psi.dat (main.c:129-131) */
   {
-    double *psi = (double *)malloc(sizeof(double) * kept * N);
+    double *psi = (double *)xmalloc(sizeof(double) * kept * N);
     if (!psi) die("psi.dat buffer");
     CHECK(sep_states_download(ctx, psi, kept));
     FILE *pf = out_file("psi.dat", "wb");

@@ -45,6 +45,19 @@ static void die(const char *msg) {
   exit(1);
 }
 
+// host buffers: a failed allocation ends the program with a message
+static void *xmalloc(size_t bytes) {
+  void *p = malloc(bytes ? bytes : 1);
+  if (!p) die("out of host memory");
+  return p;
+}
+
+static void *xcalloc(size_t n, size_t size) {
+  void *p = calloc(n ? n : 1, size);
+  if (!p) die("out of host memory");
+  return p;
+}
+
 static double wall(void) {
   struct timeval tv;
   gettimeofday(&tv, NULL);
@@ -121,9 +134,9 @@ int main(int argc, char **argv) {
   long natom = 0;
   FILE *pc = out_file("conf.par", "r");
   if (!pc || fscanf(pc, "%ld", &natom) != 1 || natom < 1) die("cannot read conf.par");
-  double *xyz = (double *)malloc(sizeof(double) * 3 * natom);
-  int *typ = (int *)malloc(sizeof(int) * natom);
-  double *vso = (double *)malloc(sizeof(double) * natom);
+  double *xyz = (double *)xmalloc(sizeof(double) * 3 * natom);
+  int *typ = (int *)xmalloc(sizeof(int) * natom);
+  double *vso = (double *)xmalloc(sizeof(double) * natom);
   long nsemicond = 0;
   double mean[3] = {0.0, 0.0, 0.0};
   for (long i = 0; i < natom; i++) {
@@ -188,8 +201,8 @@ int main(int argc, char **argv) {
   static const double la[NTYPE] = {0, 0, 0, 0, 0, 0, 0, 0, 0.64, -0.384, 0.64, -0.384, 0, 0, 0, 0};
   static const double lb[NTYPE] = {0, 0, 0, 0, 0, 0, 0, 0, 2.2287033, 2.2287033, 2.2287033,
                                    2.2287033, 0, 0, 0, 0};
-  double *vr = (double *)calloc((size_t)NTYPE * NPOT, sizeof(double));
-  double *pot = (double *)calloc((size_t)NTYPE * NPOT, sizeof(double));
+  double *vr = (double *)xcalloc((size_t)NTYPE * NPOT, sizeof(double));
+  double *pot = (double *)xcalloc((size_t)NTYPE * NPOT, sizeof(double));
   long npot[NTYPE];
   double drt[NTYPE];
   int used[NTYPE] = {0};
@@ -282,7 +295,7 @@ int main(int argc, char **argv) {
     if (pf) fclose(pf);
   }
   /* energy targets (init.c:506-513) */
-  double *el = (double *)malloc(sizeof(double) * ms);
+  double *el = (double *)xmalloc(sizeof(double) * ms);
   {
     const double del = ms > 1 ? (Elmax - Elmin) / (double)(ms - 1) : 0.0;
     FILE *pf = out_file("Egrid.dat", "w");
@@ -304,8 +317,8 @@ int main(int argc, char **argv) {
   printf("nc = %ld dt = %g dE = %g\n", nc, dt, dE);
   printf("energy range: Vmin = %.17g dE = %.17g\n", vmin, dE);
   fflush(stdout);
-  double *zn = (double *)malloc(sizeof(double) * nc);
-  double *an = (double *)malloc(sizeof(double) * nc * ms);
+  double *zn = (double *)xmalloc(sizeof(double) * nc);
+  double *an = (double *)xmalloc(sizeof(double) * nc * ms);
   sep_leja_points(nc, zn);
   if (sep_newton_coeffs(nc, ms, zn, el, dE, vmin, dt, an) != SEP_OK) die("coefficient failed");
   /* random spinors: init_psi on the up, then the down component of each,
@@ -331,8 +344,7 @@ int main(int argc, char **argv) {
   }
   printf("seed = %ld\n", idum);
   fflush(stdout);
-  double *psi0 = (double *)calloc((size_t)ns * 4 * N, sizeof(double));
-  if (!psi0) die("random spinor buffer");
+  double *psi0 = (double *)xcalloc((size_t)ns * 4 * N, sizeof(double));
   sep_rng rng;
   sep_rng_reset(&rng);
   for (long j = 0; j < ns; j++) {
@@ -348,7 +360,7 @@ int main(int argc, char **argv) {
          ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
   const long mstot = ms * ns;
   {
-    double *ene = (double *)malloc(sizeof(double) * mstot);
+    double *ene = (double *)xmalloc(sizeof(double) * mstot);
     CHECK(sep_energy_spinor(ctx, mstot, ene));
     for (long jns = 0; jns < ns; jns++) { /* filter.c:24-28 */
       char name[64];
@@ -365,7 +377,7 @@ int main(int argc, char **argv) {
   /* ortho (main.c:117-124, ortho.c:5-45) */
   t0 = wall();
   c0 = (double)clock();
-  double *S = (double *)malloc(sizeof(double) * mstot);
+  double *S = (double *)xmalloc(sizeof(double) * mstot);
   long kept = 0;
   CHECK(sep_ortho_spinor(ctx, mstot, 1.0e-10, S, &kept));
   printf("cutoff is %ld\n", kept);
@@ -375,17 +387,16 @@ int main(int argc, char **argv) {
   /* Rayleigh-Ritz (main.c:129-137, Hmat.c:5-72) */
   t0 = wall();
   c0 = (double)clock();
-  double *eval = (double *)malloc(sizeof(double) * kept);
-  double *sige = (double *)malloc(sizeof(double) * kept);
+  double *eval = (double *)xmalloc(sizeof(double) * kept);
+  double *sige = (double *)xmalloc(sizeof(double) * kept);
   CHECK(sep_rayleigh_ritz_spinor(ctx, kept, eval));
   printf("done calculating Hmat, CPU time (sec) %g, wall run time (sec) %g\n",
          ((double)clock() - c0) / CLOCKS_PER_SEC, wall() - t0);
   fflush(stdout);
   /* psi.dat: zomplex[2 ngrid] per state, up then down (main.c:139-142) */
   {
-    double *rows = (double *)malloc(sizeof(double) * 4 * N * kept);
-    double *z = (double *)malloc(sizeof(double) * 4 * N);
-    if (!rows || !z) die("psi.dat buffer");
+    double *rows = (double *)xmalloc(sizeof(double) * 4 * N * kept);
+    double *z = (double *)xmalloc(sizeof(double) * 4 * N);
     CHECK(sep_states_download(ctx, rows, 4 * kept));
     FILE *pf = out_file("psi.dat", "wb");
     if (!pf) die("cannot write psi.dat");
