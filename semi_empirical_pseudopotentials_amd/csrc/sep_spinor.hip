@@ -169,12 +169,14 @@ __global__ void __launch_bounds__(SO_T) k_so_kmat(SoLists so) {
 // points p per thread.  A 1-D grid over ceil(natom / 8) * 8 * nsp blocks:
 // block b runs on XCD b % 8 (round-robin dispatch), and the blocks of one
 // atom share that XCD and sit close in the dispatch order, so the atom's K
-// is read from HBM once and from that XCD's L2 by its other spinors.  The
-// kernel is bound by that stream of K: at 96^3 with 465 atoms (244-277
-// points each) KT = 256, NP = 1 takes 236 us per call on 8 spinors, and
-// KT = 128, NP = 2 (half the LDS bytes per FMA, but more atoms in flight
-// per XCD) 348 us.  Each point's sums run over q in the order of
-// k_so_contrib: the two kernels agree bit for bit.
+// is read from HBM once and from that XCD's L2 by its other spinors.
+// Measured at 96^3 with 465 atoms (244-277 points each), 8 spinors: KT =
+// 256, NP = 1 209 us per call (26 TFLOP/s FP64); KT = 128, NP = 2 (half the
+// LDS bytes per FMA) 348 us; one wave per spinor with four spinors at 64
+// shared points per workgroup (K from L2 a quarter as often) 216 us --
+// neither the LDS bytes per FMA nor the L2 stream of K sets its pace.  Each
+// point's sums run over q in the order of k_so_contrib: the two kernels
+// agree bit for bit.
 template <int KT, int NP>
 __global__ void __launch_bounds__(KT) k_so_contrib_k(SoLists so, const double *in, long cs,
                                                      long ss, double *contrib, long nsp) {
@@ -239,9 +241,11 @@ __global__ void __launch_bounds__(KT) k_so_contrib_k(SoLists so, const double *i
 int so_kmat_build(SoLists &so, const std::vector<long> &off, hipStream_t st) {
   if (so.natom == 0) return SEP_OK;
   std::vector<long> koff((size_t)so.natom + 1, 0);
+  so.maxn = 0;
   for (int a = 0; a < so.natom; a++) {
     const long n = off[a + 1] - off[a];
     koff[a + 1] = koff[a] + n * n;
+    so.maxn = std::max(so.maxn, n);
   }
   const size_t bytes = sizeof(double) * (size_t)koff[so.natom];
   if (bytes > ((size_t)8 << 30)) return SEP_OK;  // the inline kernel instead
