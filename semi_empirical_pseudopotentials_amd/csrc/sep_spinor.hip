@@ -173,8 +173,11 @@ __global__ void __launch_bounds__(SO_T) k_so_kmat(SoLists so) {
 // Measured at 96^3 with 465 atoms (244-277 points each), 8 spinors: KT =
 // 256, NP = 1 209 us per call (26 TFLOP/s FP64); KT = 128, NP = 2 (half the
 // LDS bytes per FMA) 348 us; one wave per spinor with four spinors at 64
-// shared points per workgroup (K from L2 a quarter as often) 216 us --
-// neither the LDS bytes per FMA nor the L2 stream of K sets its pace.  Each
+// shared points per workgroup (K from L2 a quarter as often) 216 us; the
+// 12 products per q written to memory first (44 us) and read back at
+// wave-uniform addresses through scalar loads, with no LDS (62 VGPRs,
+// occupancy 8) 246 us -- neither the LDS bytes per FMA nor the L2 stream
+// of K sets its pace.  Each
 // point's sums run over q in the order of k_so_contrib: the two kernels
 // agree bit for bit.
 template <int KT, int NP>
