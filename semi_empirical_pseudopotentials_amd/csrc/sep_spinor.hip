@@ -176,8 +176,9 @@ __global__ void __launch_bounds__(SO_T) k_so_kmat(SoLists so) {
 // shared points per workgroup (K from L2 a quarter as often) 216 us; the
 // 12 products per q written to memory first (44 us) and read back at
 // wave-uniform addresses through scalar loads, with no LDS (62 VGPRs,
-// occupancy 8) 246 us -- neither the LDS bytes per FMA nor the L2 stream
-// of K sets its pace.  Each
+// occupancy 8) 246 us; K of the next 8 q's loaded while the current 8 are
+// consumed 199 against 208 us (not kept: no gain in the filter's time) --
+// neither the LDS bytes per FMA nor the L2 stream of K sets its pace.  Each
 // point's sums run over q in the order of k_so_contrib: the two kernels
 // agree bit for bit.
 template <int KT, int NP>
