@@ -1658,6 +1658,41 @@ extern "C" int sep_refine(sep_ctx *c, long m, double *eval, double *sige) {
   return keep == m ? SEP_OK : fail(c, SEP_ERR_NUMERIC, "refine: basis of %ld < %ld states", keep, m);
 }
 
+// Small dense products of the window passes (dimensions up to a few
+// hundred): C[i ldc + j] = sum_l A[i sai + l sal] B[l sbl + j sbj], the sum
+// over l in ascending order; strides give the transposes.
+__global__ void k_small_mm(const double *A, long sai, long sal, const double *B, long sbl,
+                           long sbj, double *C, long ldc, long m, long n, long kk) {
+  const long i = (long)blockIdx.y * 16 + threadIdx.y, j = (long)blockIdx.x * 16 + threadIdx.x;
+  if (i >= m || j >= n) return;
+  double acc = 0.0;
+  for (long l = 0; l < kk; l++) acc = fma(A[i * sai + l * sal], B[l * sbl + j * sbj], acc);
+  C[i * ldc + j] = acc;
+}
+static int small_mm(sep_ctx *c, const double *A, long sai, long sal, const double *B, long sbl,
+                    long sbj, double *C, long ldc, long m, long n, long kk) {
+  hipLaunchKernelGGL(k_small_mm, dim3((unsigned)((n + 15) / 16), (unsigned)((m + 15) / 16)),
+                     dim3(16, 16), 0, c->st, A, sai, sal, B, sbl, sbj, C, ldc, m, n, kk);
+  HIPCHK(c, hipGetLastError());
+  return SEP_OK;
+}
+// H <- (H + H^T) / 2 for the r x r row-major H
+__global__ void k_symmetrize(double *H, long r) {
+  const long i = (long)blockIdx.y * 16 + threadIdx.y, j = (long)blockIdx.x * 16 + threadIdx.x;
+  if (i >= r || j >= i) return;
+  const double v = 0.5 * (H[i * r + j] + H[j * r + i]);
+  H[i * r + j] = v;
+  H[j * r + i] = v;
+}
+// s[j] = sum_i C[i][j] QC[i][j] over the nz rows of the nz x r row-major C, QC
+__global__ void k_col_dots(const double *C, const double *QC, long nz, long r, double *s) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= r) return;
+  double acc = 0.0;
+  for (long i = 0; i < nz; i++) acc = fma(C[i * r + j], QC[i * r + j], acc);
+  s[j] = acc;
+}
+
 // One correction pass of sep_refine_window on the k pairs at the front of
 // the state block (X in states rows 0..k, H X in states2 rows 0..k, energies
 // ev): Rayleigh-Ritz on Z = [X, T, X_prev] (T = P R, R = H X - e X, X_prev
@@ -1693,15 +1728,16 @@ static int window_pass(sep_ctx *c, long base, long k, double *ev, double *sg, do
   // this pass's X and H X become the next pass's previous iterate
   HIPCHK(c, hipMemcpyAsync(prev, Z, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(hprev, HZ, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
-  // the three Gram matrices
-  RCHK(ensure_scr(c, 3 * (size_t)nz * nz));
-  double *G_d = c->scr, *P_d = G_d + nz * nz, *Q_d = P_d + nz * nz;
+  // device scratch: the three Gram matrices, then the coefficient-space
+  // factors (r <= nz), all placed before any of them is used
+  const size_t q2 = (size_t)nz * nz;
+  RCHK(ensure_scr(c, 8 * q2 + (size_t)nz * k + (size_t)nz));
+  double *G_d = c->scr, *P_d = G_d + q2, *Q_d = P_d + q2, *B_d = Q_d + q2, *PB_d = B_d + q2;
+  double *H_d = PB_d + q2, *C_d = H_d + q2, *QC_d = C_d + q2, *M_d = QC_d + q2, *s_d = M_d + nz * k;
   RCHK(sub_gram_sym(c, Z, Z, nz, 1.0, G_d));
   RCHK(sub_gram_sym(c, Z, HZ, nz, 1.0, P_d));
   RCHK(sub_gram_sym(c, HZ, HZ, nz, 1.0, Q_d));
-  std::vector<double> P((size_t)nz * nz), Q((size_t)nz * nz), w(nz);
-  HIPCHK(c, hipMemcpyAsync(P.data(), P_d, sizeof(double) * nz * nz, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(c, hipMemcpyAsync(Q.data(), Q_d, sizeof(double) * nz * nz, hipMemcpyDeviceToHost, c->st));
+  std::vector<double> w(nz);
   RCHK(dev_symeig(c, nz, G_d, w.data(), "refine: Gram"));  // column j of G_d: vector j
   std::vector<double> W((size_t)nz * nz);
   HIPCHK(c, hipMemcpyAsync(W.data(), G_d, sizeof(double) * nz * nz, hipMemcpyDeviceToHost, c->st));
@@ -1712,59 +1748,31 @@ static int window_pass(sep_ctx *c, long base, long k, double *ev, double *sg, do
     if (w[i] > floor2 * lmax) dir.push_back(i);
   const long r = (long)dir.size();
   if (r < k) return fail(c, SEP_ERR_NUMERIC, "refine: basis of %ld < %ld states", r, k);
-  // B = W_r L_r^-1/2 (nz x r), reduced H = B^T P B (r x r)
-  std::vector<double> B((size_t)nz * r), PB((size_t)nz * r), Hr((size_t)r * r);
+  // B = W_r L_r^-1/2 (nz x r), reduced H = B^T P B (r x r), on the device
+  std::vector<double> B((size_t)nz * r);
   for (long j = 0; j < r; j++) {
     const double sc = 1.0 / sqrt(w[dir[j]]);
     for (long i = 0; i < nz; i++) B[i * r + j] = W[dir[j] * nz + i] * sc;
   }
-  // products with unit-stride inner loops (rows of the row-major factors)
-  for (long i = 0; i < nz; i++) {
-    double *out = &PB[i * r];
-    for (long l = 0; l < nz; l++) {
-      const double a = P[i * nz + l], *b = &B[l * r];
-      for (long j = 0; j < r; j++) out[j] += a * b[j];
-    }
-  }
-  for (long l = 0; l < nz; l++) {
-    const double *b = &B[l * r], *pb = &PB[l * r];
-    for (long i = 0; i < r; i++) {
-      const double bi = b[i];
-      double *h = &Hr[i * r];
-      for (long j = 0; j < r; j++) h[j] += bi * pb[j];
-    }
-  }
-  for (long i = 0; i < r; i++)
-    for (long j = 0; j < i; j++) Hr[i * r + j] = Hr[j * r + i] = 0.5 * (Hr[i * r + j] + Hr[j * r + i]);
+  HIPCHK(c, hipMemcpyAsync(B_d, B.data(), sizeof(double) * nz * r, hipMemcpyHostToDevice, c->st));
+  RCHK(small_mm(c, P_d, nz, 1, B_d, r, 1, PB_d, r, nz, r, nz));  // PB = P B
+  RCHK(small_mm(c, B_d, 1, r, PB_d, r, 1, H_d, r, r, r, nz));    // H = B^T PB
+  hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)((r + 15) / 16), (unsigned)((r + 15) / 16)),
+                     dim3(16, 16), 0, c->st, H_d, r);
+  HIPCHK(c, hipGetLastError());
   std::vector<double> e(r);
-  RCHK(ensure_scr(c, (size_t)r * r));
-  HIPCHK(c, hipMemcpyAsync(c->scr, Hr.data(), sizeof(double) * r * r, hipMemcpyHostToDevice, c->st));
-  RCHK(dev_symeig(c, r, c->scr, e.data(), "refine: reduced H"));
-  std::vector<double> V((size_t)r * r);
-  HIPCHK(c, hipMemcpyAsync(V.data(), c->scr, sizeof(double) * r * r, hipMemcpyDeviceToHost, c->st));
+  RCHK(dev_symeig(c, r, H_d, e.data(), "refine: reduced H"));  // column j of H_d: vector j
+  // C = B V (nz x r): coefficients of the Ritz vectors over Z; their
+  // variance c_j^T Q c_j - e_j^2 (V column-major: C[i][j] = sum_l B[i][l] V[j][l])
+  RCHK(small_mm(c, B_d, r, 1, H_d, 1, r, C_d, r, nz, r, r));
+  RCHK(small_mm(c, Q_d, nz, 1, C_d, r, 1, QC_d, r, nz, r, nz));
+  hipLaunchKernelGGL(k_col_dots, dim3((unsigned)((r + 63) / 64)), dim3(64), 0, c->st, C_d, QC_d,
+                     nz, r, s_d);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> C((size_t)nz * r), s(r);
+  HIPCHK(c, hipMemcpyAsync(C.data(), C_d, sizeof(double) * nz * r, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(s.data(), s_d, sizeof(double) * r, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  // C = B V (nz x r): coefficients of the Ritz vectors over Z; their variance
-  // (V column-major: vector j at V[j * r ..]; C[i][j] = sum_l B[i][l] V[j][l])
-  std::vector<double> C((size_t)nz * r), QC((size_t)nz * r), s(r);
-  for (long i = 0; i < nz; i++) {
-    const double *b = &B[i * r];
-    for (long j = 0; j < r; j++) {
-      const double *v = &V[j * r];
-      double acc = 0.0;
-      for (long l = 0; l < r; l++) acc += b[l] * v[l];
-      C[i * r + j] = acc;
-    }
-  }
-  for (long i = 0; i < nz; i++) {
-    double *out = &QC[i * r];
-    for (long l = 0; l < nz; l++) {
-      const double q = Q[i * nz + l], *cl = &C[l * r];
-      for (long j = 0; j < r; j++) out[j] += q * cl[j];
-    }
-  }
-  for (long j = 0; j < r; j++) s[j] = 0.0;
-  for (long i = 0; i < nz; i++)
-    for (long j = 0; j < r; j++) s[j] += C[i * r + j] * QC[i * r + j];
   for (long j = 0; j < r; j++) s[j] = sqrt(fabs(s[j] - e[j] * e[j]));
   std::vector<long> idx(r);
   for (long i = 0; i < r; i++) idx[i] = i;
@@ -1785,10 +1793,9 @@ static int window_pass(sep_ctx *c, long base, long k, double *ev, double *sg, do
     ev[j] = e[idx[j]];
     sg[j] = s[idx[j]];
   }
-  RCHK(ensure_scr(c, (size_t)nz * k));
-  HIPCHK(c, hipMemcpyAsync(c->scr, M.data(), sizeof(double) * nz * k, hipMemcpyHostToDevice, c->st));
-  RCHK(sub_combine(c, Z, nz, c->scr, k, k, Z + 3 * k * N));
-  RCHK(sub_combine(c, HZ, nz, c->scr, k, k, HZ + 3 * k * N));
+  HIPCHK(c, hipMemcpyAsync(M_d, M.data(), sizeof(double) * nz * k, hipMemcpyHostToDevice, c->st));
+  RCHK(sub_combine(c, Z, nz, M_d, k, k, Z + 3 * k * N));
+  RCHK(sub_combine(c, HZ, nz, M_d, k, k, HZ + 3 * k * N));
   HIPCHK(c, hipMemcpyAsync(Z, Z + 3 * k * N, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(HZ, HZ + 3 * k * N, sizeof(double) * k * N, hipMemcpyDeviceToDevice,
                            c->st));
