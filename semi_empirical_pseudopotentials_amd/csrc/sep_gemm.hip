@@ -386,14 +386,20 @@ __global__ void __launch_bounds__(256) k_combine_t(const R *__restrict__ A, long
       }
     }
   };
+  // FP64: the 16-row sub-tiles x of this wave's j tile that hold any state
+  // (all four unless the wave covers the tail of mo, as the second wave does
+  // for a 65-state window); the rest are not multiplied
+  const int xact = PERM ? 4 : (int)((mo - j0 + 15) / 16 < 4 ? (mo - j0 + 15) / 16 : 4);
   auto step = [&](int st) {
 #pragma unroll
     for (int s = 0; s < 4; s++)
 #pragma unroll
       for (int x = 0; x < 4; x++)
+        if (x < xact) {
 #pragma unroll
-        for (int y = 0; y < TG; y++)
-          acc[x][y] = Mfma<R>::op(mv[st][x][s], av[st][y][s], acc[x][y]);
+          for (int y = 0; y < TG; y++)
+            acc[x][y] = Mfma<R>::op(mv[st][x][s], av[st][y][s], acc[x][y]);
+        }
   };
   // the same branch-free pipeline as the Gram's: at the loop head stage
   // u < ST - 1 holds i-block i0 + u GKB; blocks past the end re-read block 0
