@@ -1702,32 +1702,23 @@ __global__ void k_col_dots(const double *C, const double *QC, long nz, long r, d
 // diagonalised, sigma^2 = c^T (HZ HZ^T) c - e^2 -- and only the k kept pairs
 // (inside [lo, hi] first, then by variance) rotated, with their H psi, both
 // normalized as filter/main.c:122.  One H application per pass (on T).
-// On return prev / hprev hold this pass's X and H X.  X and H X are rows
-// base .. base + k of the two state buffers, which hold base + 4k rows.
+// X and H X are rows base .. base + k of the two state buffers, which hold
+// base + 4k rows: T and H T follow at rows k .. 2k, X_prev and H X_prev
+// (the previous pass's X, written there by that pass when keep_prev) at
+// 2k .. 3k, and rows 3k .. 4k are scratch.
 static int window_pass(sep_ctx *c, long base, long k, double *ev, double *sg, double lo,
-                       double hi, double *prev, double *hprev, bool have_prev) {
+                       double hi, bool have_prev, bool keep_prev) {
   const long N = c->gd.N;
   const long nz = have_prev ? 3 * k : 2 * k;
   double *Z = c->states + base * N, *HZ = c->states2 + base * N;
-  // T = P (H X - e X), normalized, into Z rows k..2k; H T into HZ rows k..2k
+  // R = H X - e X into rows 3k..4k; T = P R, normalized, into rows k..2k;
+  // H T into HZ rows k..2k
   RCHK(ensure_scr(c, k));
-  HIPCHK(c, hipMemcpyAsync(Z + k * N, HZ, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(c->scr, ev, sizeof(double) * k, hipMemcpyHostToDevice, c->st));
-  RCHK(sub_scaled_rows_dev(Z + k * N, N, Z, N, k, N, c->scr, c->st));
-  RCHK(precond_states(c, Z + k * N, Z + 3 * k * N, k, SEP_REFINE_SHIFT));
-  HIPCHK(c, hipMemcpyAsync(Z + k * N, Z + 3 * k * N, sizeof(double) * k * N,
-                           hipMemcpyDeviceToDevice, c->st));
+  RCHK(diff_scaled_rows_dev(Z + 3 * k * N, N, HZ, N, Z, N, k, N, c->scr, c->st));
+  RCHK(precond_states(c, Z + 3 * k * N, Z + k * N, k, SEP_REFINE_SHIFT));
   RCHK(normalize_dev(c, Z + k * N, k));
   RCHK(apply_states(c, Z + k * N, HZ + k * N, k, nullptr, nullptr));
-  if (have_prev) {
-    HIPCHK(c, hipMemcpyAsync(Z + 2 * k * N, prev, sizeof(double) * k * N, hipMemcpyDeviceToDevice,
-                             c->st));
-    HIPCHK(c, hipMemcpyAsync(HZ + 2 * k * N, hprev, sizeof(double) * k * N,
-                             hipMemcpyDeviceToDevice, c->st));
-  }
-  // this pass's X and H X become the next pass's previous iterate
-  HIPCHK(c, hipMemcpyAsync(prev, Z, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(c, hipMemcpyAsync(hprev, HZ, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
   // device scratch: the three Gram matrices, then the coefficient-space
   // factors (r <= nz), all placed before any of them is used
   const size_t q2 = (size_t)nz * nz;
@@ -1796,6 +1787,12 @@ static int window_pass(sep_ctx *c, long base, long k, double *ev, double *sg, do
   HIPCHK(c, hipMemcpyAsync(M_d, M.data(), sizeof(double) * nz * k, hipMemcpyHostToDevice, c->st));
   RCHK(sub_combine(c, Z, nz, M_d, k, k, Z + 3 * k * N));
   RCHK(sub_combine(c, HZ, nz, M_d, k, k, HZ + 3 * k * N));
+  if (keep_prev) {  // this pass's X and H X: the next pass's X_prev
+    HIPCHK(c, hipMemcpyAsync(Z + 2 * k * N, Z, sizeof(double) * k * N, hipMemcpyDeviceToDevice,
+                             c->st));
+    HIPCHK(c, hipMemcpyAsync(HZ + 2 * k * N, HZ, sizeof(double) * k * N,
+                             hipMemcpyDeviceToDevice, c->st));
+  }
   HIPCHK(c, hipMemcpyAsync(Z, Z + 3 * k * N, sizeof(double) * k * N, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(HZ, HZ + 3 * k * N, sizeof(double) * k * N, hipMemcpyDeviceToDevice,
                            c->st));
@@ -1828,13 +1825,8 @@ extern "C" int sep_refine_window(sep_ctx *c, long m, double *eval, double *sige,
                              hipMemcpyDeviceToDevice, c->st));
   }
   RCHK(apply_states(c, X, c->states2 + m * N, k, nullptr, nullptr));  // H X
-  double *prev = nullptr;
-  HIPCHK(c, dev_malloc(c, (void **)&prev, sizeof(double) * 2 * k * N));
-  int rc = SEP_OK;
-  for (int p = 0; p < passes && rc == SEP_OK; p++)
-    rc = window_pass(c, m, k, ev.data(), sg.data(), lo, hi, prev, prev + k * N, p > 0);
-  hipFree(prev);
-  RCHK(rc);
+  for (int p = 0; p < passes; p++)
+    RCHK(window_pass(c, m, k, ev.data(), sg.data(), lo, hi, p > 0, p + 1 < passes));
   // the refined pairs (ascending) back into the window's positions
   for (long j = 0; j < k; j++) {
     HIPCHK(c, hipMemcpyAsync(c->states + idx[j] * N, X + j * N, sizeof(double) * N,

@@ -189,6 +189,9 @@ int hf_adjoint_dev(const GridDev &gd, double xmin, double ymin, double zmin, dou
 
 int sub_scaled_rows_dev(double *y, long ldy, const double *x, long ldx, long m, long n,
                         const double *s_d, hipStream_t st);
+// out_i = y_i - s_i x_i (row i, n entries) for m rows
+int diff_scaled_rows_dev(double *out, long ldo, const double *y, long ldy, const double *x,
+                         long ldx, long m, long n, const double *s_d, hipStream_t st);
 int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStream_t st);
 int convert_d2f(const double *in, float *out, long n, hipStream_t st);
 int convert_f2d(const float *in, double *out, long n, hipStream_t st);

@@ -152,20 +152,25 @@ int scale_rows_dev(double *x, long ld, long m, long n, const double *s_d, hipStr
 }
 
 // y_i -= s_i x_i for m rows of n points (residuals H psi - e psi)
-__global__ void k_sub_scaled_rows(double *y, long ldy, const double *x, long ldx, long n,
-                                  const double *s) {
+// out_i = y_i - s_i x_i for the m rows (out may be y)
+__global__ void k_sub_scaled_rows(double *out, long ldo, const double *y, long ldy,
+                                  const double *x, long ldx, long n, const double *s) {
   const long i = blockIdx.y;
   const double f = s[i];
   for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
-    y[i * ldy + g] -= f * x[i * ldx + g];
+    out[i * ldo + g] = y[i * ldy + g] - f * x[i * ldx + g];
+}
+int diff_scaled_rows_dev(double *out, long ldo, const double *y, long ldy, const double *x,
+                         long ldx, long m, long n, const double *s_d, hipStream_t st) {
+  unsigned nbx = (unsigned)((n + 255) / 256);
+  if (nbx > 4096) nbx = 4096;
+  hipLaunchKernelGGL(k_sub_scaled_rows, dim3(nbx, (unsigned)m), dim3(256), 0, st, out, ldo, y, ldy,
+                     x, ldx, n, s_d);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 int sub_scaled_rows_dev(double *y, long ldy, const double *x, long ldx, long m, long n,
                         const double *s_d, hipStream_t st) {
-  unsigned nbx = (unsigned)((n + 255) / 256);
-  if (nbx > 4096) nbx = 4096;
-  hipLaunchKernelGGL(k_sub_scaled_rows, dim3(nbx, (unsigned)m), dim3(256), 0, st, y, ldy, x, ldx,
-                     n, s_d);
-  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+  return diff_scaled_rows_dev(y, ldy, y, ldy, x, ldx, m, n, s_d, st);
 }
 
 __global__ void k_d2f(const double *in, float *out, long n) {
