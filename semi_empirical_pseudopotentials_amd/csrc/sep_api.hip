@@ -335,7 +335,7 @@ int sep_close(sep_ctx *c) {
     if (e) hipEventDestroy(e);
   for (void *p : {(void *)c->so.off, (void *)c->so.jxyz, (void *)c->so.geo, (void *)c->so.pidx,
                   (void *)c->so.poff, (void *)c->so.pent, (void *)c->so.kmat, (void *)c->so.koff,
-                  (void *)c->so_contrib})
+                  (void *)c->so.pmap, (void *)c->so_contrib})
     if (p) hipFree(p);
   if (c->blas) rocblas_destroy_handle(c->blas);
   if (c->dws) hipFree(c->dws);
@@ -842,7 +842,11 @@ static int filter_start(FilterRun<R> &f, sep_ctx *c, const double *psi0, bool ps
   f.j = 1;
   // pass A of psi_0; every later pass A runs fused with the previous pass C
   f.so = so;
-  f.fused = !so && nc > 1 && !c->gd.no_fuse && hpsi_fused_supported<R>(c->gd);
+  // spinors: V_so joins psi_j in the fused kernel's epilogue where the FP64
+  // whole-plane family runs; elsewhere the steps run unfused
+  const bool so_fused = sizeof(R) == 8 && c->so.pmap && c->so.natom > 0 &&
+                        hpsi_plane_family(c->gd, SEP_PREC_FP64);
+  f.fused = (!so || so_fused) && nc > 1 && !c->gd.no_fuse && hpsi_fused_supported<R>(c->gd);
   if (f.fused) {
     VecView<R> v0{f.ring, f.ring + ss, nullptr, 2 * ss};
     if (hpsi_fwd<R>(c->gd, v0, f.tb, (int)f.nvec, f.st) != SEP_OK)
@@ -885,9 +889,36 @@ static int filter_step(FilterRun<R> &f) {
   const double beta = 2.0 + f.zn[j - 1] + f.vmin * f.dE_1;
   const bool sample = c->timing && sj == K / 2;
   if (f.fused) {
-    if (hpsi_step<R>(c->gd, in, out, f.tb, pot_of<R>(c), (int)f.nvec, f.dE_1, beta, j + 1 < f.nc,
-                     f.st, sample ? c->tev : nullptr) != SEP_OK)
+    const bool fwd_next = j + 1 < f.nc;
+    SoEpi se{}, *sep_ = nullptr;
+    if constexpr (sizeof(R) == 8) {
+      if (f.so && c->so.natom > 0) {
+        // V_so psi_{j-1} per covered point, added in the epilogue; the last
+        // step (no fused forward pass) adds it after the step as below
+        const long nsp = f.ns / 4;
+        const size_t nc4 = 4 * (size_t)c->so.nent * nsp;
+        RCHK(ensure(c, (void **)&c->so_contrib, &c->so_contrib_bytes,
+                    sizeof(double) * (nc4 + 4 * (size_t)c->so.npts * nsp)));
+        if (fwd_next) {
+          if (so_gather_dev(c->so, f.ring + sp * slot, ss, 4 * ss, nsp, c->so_contrib,
+                            c->so_contrib + nc4, f.st) != SEP_OK)
+            return fail(c, SEP_ERR_HIP, "filter: spin-orbit launch failed");
+          se = SoEpi{c->so.pmap, c->so_contrib + nc4, nsp, f.dE_1};
+          sep_ = &se;
+        }
+      }
+    }
+    if (hpsi_step<R>(c->gd, in, out, f.tb, pot_of<R>(c), (int)f.nvec, f.dE_1, beta, fwd_next,
+                     f.st, sample ? c->tev : nullptr, sep_) != SEP_OK)
       return fail(c, SEP_ERR_HIP, "filter: H.psi step launch failed");
+    if constexpr (sizeof(R) == 8) {
+      if (f.so && c->so.natom > 0 && !fwd_next) {
+        const long nsp = f.ns / 4;
+        if (so_apply_dev(c->so, f.ring + sp * slot, ss, 4 * ss, f.ring + sj * slot, ss, 4 * ss,
+                         nsp, f.dE_1, c->so_contrib, f.st) != SEP_OK)
+          return fail(c, SEP_ERR_HIP, "filter: spin-orbit launch failed");
+      }
+    }
   } else {
     RCHK(apply_views<R>(c, in, out, (int)f.nvec, f.dE_1, beta, nullptr, sample ? c->tev : nullptr));
     if constexpr (sizeof(R) == 8) {
@@ -951,7 +982,8 @@ static int filter_finish(FilterRun<R> &f, int rc) {
 // default priority.
 template <typename R>
 static int filter_lanes(sep_ctx *c, const FilterRun<R> &f) {
-  if (!f.fused || c->timing || f.nc <= 1) return 1;
+  // spinors: one lane (the spin-orbit work buffer is shared)
+  if (!f.fused || f.so || c->timing || f.nc <= 1) return 1;
   long L = c->lanes > 0 ? c->lanes : (f.nvec >= 4 ? 2 : 1);
   if (L > f.nvec) L = f.nvec;
   for (long l = 0; l < L; l++) {
@@ -2507,7 +2539,8 @@ int sep_so_setup(sep_ctx *c, long natom, const double *xyz, const double *vso, l
   const double d[3] = {c->dx, c->dy, c->dz}, x0[3] = {c->xmin, c->ymin, c->zmin};
   RCHK(so_build_host(c->gd.nx, c->gd.ny, c->gd.nz, d, x0, natom, xyz, vso, h));
   for (void *p : {(void *)c->so.off, (void *)c->so.jxyz, (void *)c->so.geo, (void *)c->so.pidx,
-                  (void *)c->so.poff, (void *)c->so.pent, (void *)c->so.kmat, (void *)c->so.koff})
+                  (void *)c->so.poff, (void *)c->so.pent, (void *)c->so.kmat, (void *)c->so.koff,
+                  (void *)c->so.pmap})
     if (p) hipFree(p);
   c->so = SoLists();
   SoLists &s = c->so;
@@ -2533,6 +2566,7 @@ int sep_so_setup(sep_ctx *c, long natom, const double *xyz, const double *vso, l
   RCHK(up((void **)&s.poff, h.poff.data(), sizeof(long) * h.poff.size()));
   RCHK(up((void **)&s.pent, h.pent.data(), sizeof(int) * h.pent.size()));
   if (so_kmat_build(s, h.off, c->st) != SEP_OK) return fail(c, SEP_ERR_HIP, "so_setup: K matrices");
+  if (so_pmap_build(s, c->gd.N, c->st) != SEP_OK) return fail(c, SEP_ERR_HIP, "so_setup: point map");
   HIPCHK(c, hipStreamSynchronize(c->st));
   return SEP_OK;
 }

@@ -23,6 +23,13 @@ struct Epi {
   const R *pot;   // V on the grid
   double alpha, beta;
   double *part;   // per-block partial sums (4 per block) or nullptr
+  // spinor filter (FP64 whole-plane fused kernel only): psi_j += so_alpha *
+  // (V_so psi_{j-1}) at the grid points in a spin-orbit list, so_map[g] the
+  // point's row of so_g (4 doubles per spinor: up re, im, dn re, im) or -1
+  const int *so_map = nullptr;
+  const double *so_g = nullptr;
+  long so_nsp = 0;
+  double so_alpha = 0.0;
 };
 
 // block reduction of 4 doubles; writes part[blk*4 + q]
@@ -875,6 +882,16 @@ __global__ void __launch_bounds__(T) k_plane_bwd_fwd_w(cx<R> *tb, VecView<R> pin
         for (int k = 0; k < rlb; k++) {
           const long gi = pbase + rs * NX + out_idx<NX, 1, P - 1>(g, b, k);
           x[b * rlb + k] = epilogue(epi, x[b * rlb + k], p[b * rlb + k], V[b * rlb + k], s);
+          if constexpr (sizeof(R) == 8) {
+            if (epi.so_map) {  // uniform: spinor filters only
+              const int u = epi.so_map[gi];
+              if (u >= 0) {  // vector v: spinor v / 2, up (v even) or down
+                const double *gq = epi.so_g + ((long)u * epi.so_nsp + (vec >> 1)) * 4 + 2 * (vec & 1);
+                x[b * rlb + k].re = fma(epi.so_alpha, gq[0], (double)x[b * rlb + k].re);
+                x[b * rlb + k].im = fma(epi.so_alpha, gq[1], (double)x[b * rlb + k].im);
+              }
+            }
+          }
           out.st(vec, gi, x[b * rlb + k]);
         }
     }
@@ -1514,7 +1531,7 @@ static bool plane_bwd_fwd_launch(cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<
                                  hipStream_t st, int form) {
   if constexpr (plane_colmajor<n>() && plane_chunks<R, n>() == 1) {
     if constexpr (plane_fused_s<R, n>()) {
-      if (form == SEP_FUSED_PLANE && !epi.part && tws) {
+      if (form == SEP_FUSED_PLANE && !epi.part && !epi.so_map && tws) {
         constexpr int T = 1024;
         constexpr size_t lds = plane_sep_lds();
         static_assert(lds <= 160 * 1024, "plane buffer exceeds the LDS of a gfx950 workgroup");
@@ -1531,7 +1548,8 @@ static bool plane_bwd_fwd_launch(cx<R> *tb, VecView<R> pin, VecView<R> out, Epi<
       }
     }
     if constexpr (plane_fused_h<R, n>()) {
-      if (form == SEP_FUSED_CHUNKED && !epi.part && !pin.c && pin.im && !out.c && out.im) {
+      if (form == SEP_FUSED_CHUNKED && !epi.part && !epi.so_map && !pin.c && pin.im && !out.c &&
+          out.im) {
         constexpr int T = 512, NH = 2;
         constexpr size_t lds = sizeof(cx<R>) * ((size_t)n * PlaneLds<R, n, NH>::LDc + n);
         static_assert(2 * lds <= 160 * 1024, "two chunk buffers must share the LDS of a CU");
@@ -1808,7 +1826,7 @@ int hpsi_fwd(const GridDev &gd, const VecView<R> &in, cx<R> *tb, int nvec, hipSt
 template <typename R>
 int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
               const R *pot, int nvec, double alpha, double beta, bool fwd_next, hipStream_t st,
-              hipEvent_t *ev) {
+              hipEvent_t *ev, const SoEpi *so) {
   const int nx = gd.nx, ny = gd.ny, nz = gd.nz;
   const long N = gd.N;
   const int prec = sizeof(R) == 8 ? SEP_PREC_FP64 : SEP_PREC_FP32;
@@ -1817,6 +1835,8 @@ int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx
   const cx<R> *twz = reinterpret_cast<const cx<R> *>(sizeof(R) == 8 ? gd.tw64[2] : gd.tw32[2]);
   if (!hpsi_fused_supported<R>(gd) || !in.im || !out.im) return SEP_ERR_UNSUPPORTED;
   const bool plane = split_plane(gd, prec);
+  // the spin-orbit add rides only in the FP64 whole-plane fused kernel
+  if (so && (sizeof(R) != 8 || !plane || !fwd_next)) return SEP_ERR_UNSUPPORTED;
   if (ev) hipEventRecord(ev[1], st);
   if (!plane && !HOps<R>::lines(ny, -1, tb, N, nx, nx, (long)nx * ny, nz, nvec, twy, st))
     return SEP_ERR_UNSUPPORTED;
@@ -1828,6 +1848,12 @@ int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx
     return SEP_ERR_UNSUPPORTED;
   if (ev) hipEventRecord(ev[2], st);
   Epi<R> epi{pot, alpha, beta, nullptr};
+  if (so) {
+    epi.so_map = so->map;
+    epi.so_g = so->g;
+    epi.so_nsp = so->nsp;
+    epi.so_alpha = so->alpha;
+  }
   bool ok;
   if (plane)
     ok = fwd_next ? HOps<R>::plane_bwd_fwd(nx, tb, in, out, epi, N, nz, nvec, twx, sep_tws<R>(gd),
@@ -1848,10 +1874,11 @@ template int hpsi_fwd<double>(const GridDev &, const VecView<double> &, cx<doubl
 template int hpsi_fwd<float>(const GridDev &, const VecView<float> &, cx<float> *, int, hipStream_t);
 template int hpsi_step<double>(const GridDev &, const VecView<double> &, const VecView<double> &,
                                cx<double> *, const double *, int, double, double, bool,
-                               hipStream_t, hipEvent_t *);
+                               hipStream_t, hipEvent_t *, const SoEpi *);
 template int hpsi_step<float>(const GridDev &, const VecView<float> &, const VecView<float> &,
                               cx<float> *, const float *, int, double, double, bool, hipStream_t,
-                              hipEvent_t *);
+                              hipEvent_t *, const SoEpi *);
+bool hpsi_plane_family(const GridDev &gd, int prec) { return split_plane(gd, prec); }
 
 // Kernel attributes of this grid's H.psi kernels, set outside any stream
 // capture (hipFuncSetAttribute is not a stream operation).

@@ -52,6 +52,7 @@ struct SoLists {
   // at kmat + koff[a] + q n; null where it would take more than 8 GB
   double *kmat = nullptr;
   long *koff = nullptr;
+  int *pmap = nullptr;  // [N] grid point -> its index u < npts, or -1
 };
 struct SoHost {
   std::vector<long> off, poff;
@@ -65,9 +66,15 @@ int so_build_host(int nx, int ny, int nz, const double d[3], const double x0[3],
 // in + s ss + c cs (out likewise); contrib holds nent * nsp * 4 doubles
 int so_apply_dev(const SoLists &so, const double *in, long cs, long ss, double *out, long ocs,
                  long oss, long nsp, double alpha, double *contrib, hipStream_t st);
+// V_so in_s gathered per covered point: g[(u nsp + s) 4 + c] for u < npts
+// (the sums of so_apply_dev, without the add); contrib as there
+int so_gather_dev(const SoLists &so, const double *in, long cs, long ss, long nsp, double *contrib,
+                  double *g, hipStream_t st);
 // Builds so.kmat / so.koff from the uploaded lists (off: the host copy of
 // so.off); leaves them null where they would exceed 8 GB.
 int so_kmat_build(SoLists &so, const std::vector<long> &off, hipStream_t st);
+// Builds so.pmap over the N grid points; leaves it null when it does not fit
+int so_pmap_build(SoLists &so, long N, hipStream_t st);
 int spinor_times_i_dev(const double *x, double *out, long N, long m, hipStream_t st);
 int spinor_gather_rows_dev(const double *src, double *dst, long N, long nsp, long ms,
                            hipStream_t st);
@@ -101,10 +108,21 @@ template <typename R>
 bool hpsi_fused_supported(const GridDev &gd);
 template <typename R>
 int hpsi_fwd(const GridDev &gd, const VecView<R> &in, cx<R> *tb, int nvec, hipStream_t st);
+// spinor filter step: psi_j += alpha (V_so psi_{j-1}) inside the fused
+// kernel's epilogue (FP64 whole-plane family, not the last step): map[g] the
+// row of g (4 doubles per spinor) in g, or -1
+struct SoEpi {
+  const int *map;
+  const double *g;
+  long nsp;
+  double alpha;
+};
 template <typename R>
 int hpsi_step(const GridDev &gd, const VecView<R> &in, const VecView<R> &out, cx<R> *tb,
               const R *pot, int nvec, double alpha, double beta, bool fwd_next, hipStream_t st,
-              hipEvent_t *ev = nullptr);
+              hipEvent_t *ev = nullptr, const SoEpi *so = nullptr);
+// whether this grid's H.psi runs the whole-plane kernel family at prec
+bool hpsi_plane_family(const GridDev &gd, int prec);
 
 // ---- cubic Hermite interpolation of a uniform radial table (the smooth
 // option of the differentiable path, SEP_INTERP_CUBIC).  On segment i

@@ -242,6 +242,45 @@ __global__ void __launch_bounds__(KT) k_so_contrib_k(SoLists so, const double *i
   }
 }
 
+// k_so_gather's sums without the add: g[(u nsp + s) 4 + c], the entries of
+// point u in atom order
+__global__ void k_so_gather_g(SoLists so, const double *contrib, long nsp, double *g) {
+  const long u = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long s = blockIdx.y;
+  if (u >= so.npts) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long e = so.poff[u]; e < so.poff[u + 1]; e++) {
+    const double *c = contrib + ((long)so.pent[e] * nsp + s) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += c[k];
+  }
+  double *o = g + (u * nsp + s) * 4;
+#pragma unroll
+  for (int k = 0; k < 4; k++) o[k] = acc[k];
+}
+
+// so.pmap: -1 everywhere, then u at the grid index of covered point u
+__global__ void k_so_pmap_fill(int *pmap, long n) {
+  for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+    pmap[g] = -1;
+}
+__global__ void k_so_pmap_set(SoLists so) {
+  const long u = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < so.npts) so.pmap[so.pidx[u]] = (int)u;
+}
+
+int so_pmap_build(SoLists &so, long N, hipStream_t st) {
+  if (so.natom == 0 || so.npts == 0) return SEP_OK;
+  if (hipMalloc(&so.pmap, sizeof(int) * N) != hipSuccess) {
+    (void)hipGetLastError();
+    so.pmap = nullptr;  // the spinor filter then runs its steps unfused
+    return SEP_OK;
+  }
+  hipLaunchKernelGGL(k_so_pmap_fill, dim3(1024), dim3(256), 0, st, so.pmap, N);
+  hipLaunchKernelGGL(k_so_pmap_set, dim3((unsigned)((so.npts + 255) / 256)), dim3(256), 0, st, so);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
 int so_kmat_build(SoLists &so, const std::vector<long> &off, hipStream_t st) {
   if (so.natom == 0) return SEP_OK;
   std::vector<long> koff((size_t)so.natom + 1, 0);
@@ -268,9 +307,8 @@ int so_kmat_build(SoLists &so, const std::vector<long> &off, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
 }
 
-int so_apply_dev(const SoLists &so, const double *in, long cs, long ss, double *out, long ocs,
-                 long oss, long nsp, double alpha, double *contrib, hipStream_t st) {
-  if (so.natom == 0 || nsp < 1) return SEP_OK;
+static void so_contrib_launch(const SoLists &so, const double *in, long cs, long ss, long nsp,
+                              double *contrib, hipStream_t st) {
   if (so.kmat) {
     const long nb = (long)((so.natom + 7) / 8) * 8 * nsp;
     hipLaunchKernelGGL((k_so_contrib_k<256, 1>), dim3((unsigned)nb), dim3(256), 0, st, so, in, cs,
@@ -279,6 +317,21 @@ int so_apply_dev(const SoLists &so, const double *in, long cs, long ss, double *
     hipLaunchKernelGGL(k_so_contrib, dim3((unsigned)so.natom, (unsigned)nsp), dim3(SO_T), 0, st,
                        so, in, cs, ss, contrib, nsp);
   }
+}
+
+int so_gather_dev(const SoLists &so, const double *in, long cs, long ss, long nsp, double *contrib,
+                  double *g, hipStream_t st) {
+  if (so.natom == 0 || nsp < 1) return SEP_OK;
+  so_contrib_launch(so, in, cs, ss, nsp, contrib, st);
+  hipLaunchKernelGGL(k_so_gather_g, dim3((unsigned)((so.npts + 255) / 256), (unsigned)nsp),
+                     dim3(256), 0, st, so, contrib, nsp, g);
+  return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
+}
+
+int so_apply_dev(const SoLists &so, const double *in, long cs, long ss, double *out, long ocs,
+                 long oss, long nsp, double alpha, double *contrib, hipStream_t st) {
+  if (so.natom == 0 || nsp < 1) return SEP_OK;
+  so_contrib_launch(so, in, cs, ss, nsp, contrib, st);
   hipLaunchKernelGGL(k_so_gather, dim3((unsigned)((so.npts + 255) / 256), (unsigned)nsp), dim3(256),
                      0, st, so, contrib, nsp, out, ocs, oss, alpha);
   return hipGetLastError() == hipSuccess ? SEP_OK : SEP_ERR_HIP;
